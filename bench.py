@@ -1,0 +1,230 @@
+"""Benchmark of the GPU-driven draw-cull path on one MI355X.
+
+Workload (one step = one frame of the two-pass cull): 4,194,304 objects over 256 surfaces x 8 LODs, positions
+U[-500,500]^3, scale U[0.5,2], unit quaternions; camera at the origin looking +z, 70 deg fov, 1920x1080, reverse-z,
+zNear 0.1, zFar 650; a synthetic 1920x1080 reverse-z depth buffer (ground plane y = -10 plus 512 seeded occluder
+rectangles of depth U[0.001,0.2]) reduced to a 1024x512-based 11-mip min-depth pyramid; last-frame visibility
+Bernoulli(0.5). A step runs: pyramid build -> early pass (frustum + LOD over last-frame-visible objects) -> late
+pass (frustum + Hi-Z over all objects, emits newly visible, writes visibility).
+
+Inputs are resident in HBM before timing. Steps rotate over --copies independent copies of the scene, depth buffer,
+pyramid and visibility buffers (4 x ~200 MB by default) so no step finds its inputs in the 256 MB MALL left by the
+previous one. The late pass writes visibility out of place so every step does identical work.
+
+Prints ONE JSON line (rank 0). object-culls/s = objects / (time of one full frame).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N_OBJECTS = 4194304
+N_SURFACES, N_LODS = 256, 8
+DEPTH_W, DEPTH_H, OCCLUDERS = 1920, 1080, 512
+PYR_W, PYR_H, PYR_MIPS = 1024, 512, 11
+SEED = 0xB1172E01
+HBM_PEAK_GBS = 8000.0
+LATE_BYTES_PER_OBJECT = 48  # RenderObject 8 + MeshTransform 32 + visibility read 4 + write 4
+EARLY_BYTES_PER_OBJECT = 4  # visibility read; + 40 B per last-frame-visible object
+CMD_BYTES = 24
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def ensure_built():
+    from blitzen_amd import _capi
+
+    if not _capi.LIB_PATH.exists():
+        import __graft_entry__
+
+        __graft_entry__.build()
+
+
+def cpu_baseline(scene, vb, depth, vis0, budget_s: float):
+    """The oracle (OpenMP port of the reference shaders) on the host cores, full C1 frame, bounded by budget_s."""
+    from oracle import pyoracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    runs, times = 0, []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        texels = orc.pyramid_build(depth, PYR_W, PYR_H, PYR_MIPS, threads=threads)
+        pyr = orc.pyramid(texels, PYR_W, PYR_H, PYR_MIPS)
+        orc.cull(scene, vb, 0, vis_in=vis0, threads=threads)
+        vout = vis0.copy()
+        orc.cull(scene, vb, 1, pyr=pyr, vis_in=vis0, vis_out=vout, threads=threads)
+        times.append(time.perf_counter() - t0)
+        runs += 1
+        if runs >= 3 and (time.perf_counter() - t_start > budget_s or runs >= 30):
+            break
+    med = statistics.median(times[1:]) if len(times) > 1 else times[0]
+    return {"value": N_OBJECTS / med, "unit": "object-culls/s", "cores": threads, "kind": "port",
+            "sample": f"full frame (pyramid + early + late, {N_OBJECTS} objects), median of {len(times) - 1} runs "
+                      f"after 1 warm-up, OpenMP {threads} threads",
+            "ms_per_frame": med * 1e3}
+
+
+def load_traffic():
+    p = ROOT / "profiles" / "traffic_late_cull_c1.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text())
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--copies", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    ensure_built()
+    import numpy as np
+
+    from blitzen_amd import _capi as A
+    from blitzen_amd import cull as bz
+    from blitzen_amd.hip import Event
+
+    t0 = time.perf_counter()
+    sc = bz.synth_scene(SEED, N_OBJECTS, N_SURFACES, N_LODS)
+    v = bz.camera_setup(70.0, DEPTH_W, DEPTH_H, 0.1, 650.0)
+    v.pyramidWidth, v.pyramidHeight = float(PYR_W), float(PYR_H)
+    depth = bz.synth_depth(SEED, DEPTH_W, DEPTH_H, OCCLUDERS, v.proj5, 0.1)
+    vis0 = bz.synth_visibility(SEED, N_OBJECTS, 0.5)
+    log(f"bench: workload generated in {time.perf_counter() - t0:.1f}s")
+
+    ctx = bz.Context(0)
+    name, ncu, hbm = ctx.device_info()
+    log(f"bench: device {name}, {ncu} CUs, {hbm / 2**30:.0f} GiB")
+    stream = ctx.stream()
+    copies = []
+    for c in range(args.copies):
+        h = ctx.scene_create(sc.objects, sc.transforms, sc.surfaces, sc.lods)
+        vin = ctx.alloc(N_OBJECTS * 4)
+        vin.upload(vis0)
+        vout = ctx.alloc(N_OBJECTS * 4)
+        dbuf = ctx.alloc(depth.nbytes)
+        dbuf.upload(depth)
+        pyr = ctx.pyramid_create(PYR_W, PYR_H, PYR_MIPS)
+        copies.append((h, vin, vout, dbuf, pyr))
+    cmds_e = ctx.alloc(N_OBJECTS * CMD_BYTES)
+    cmds_l = ctx.alloc(N_OBJECTS * CMD_BYTES)
+    cnt = ctx.alloc(16)
+    cnt_e_ptr, cnt_l_ptr = cnt.ptr, cnt.ptr + 4
+
+    class _Cnt:  # count sub-buffers
+        def __init__(self, p):
+            self.ptr = p
+
+    ce, cl = _Cnt(cnt_e_ptr), _Cnt(cnt_l_ptr)
+
+    def step(k, ev=None):
+        h, vin, vout, dbuf, pyr = copies[k % len(copies)]
+        if ev:
+            ev[0].record(stream)
+        ctx.pyramid_build(pyr, dbuf.ptr, DEPTH_W, DEPTH_H, stream)
+        if ev:
+            ev[1].record(stream)
+        ctx.cull(h, v, A.PASS_EARLY, cmds_e, ce, capacity=N_OBJECTS, vis_in=vin.ptr, stream=stream)
+        if ev:
+            ev[2].record(stream)
+        ctx.cull(h, v, A.PASS_LATE, cmds_l, cl, capacity=N_OBJECTS, pyramid=pyr, vis_in=vin.ptr, vis_out=vout.ptr,
+                 stream=stream)
+        if ev:
+            ev[3].record(stream)
+
+    for k in range(args.warmup):
+        step(k)
+    ctx.check_kernels(stream)
+    counts = cnt.download(np.uint32, 2, stream)
+    n_early, n_late = int(counts[0]), int(counts[1])
+
+    # timed region: wall clock over exactly K steps, device synchronized on both sides
+    ctx.sync(stream)
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    ctx.sync(stream)
+    wall = time.perf_counter() - t_start
+    ctx.check_kernels(stream)
+
+    # per-kernel durations with HIP events on the launch stream (separate pass, same work)
+    evs = [[Event() for _ in range(4)] for _ in range(args.steps)]
+    for k in range(args.steps):
+        step(k, evs[k])
+    ctx.sync(stream)
+    t_pyr = [e[0].elapsed_ms(e[1]) for e in evs]
+    t_early = [e[1].elapsed_ms(e[2]) for e in evs]
+    t_late = [e[2].elapsed_ms(e[3]) for e in evs]
+    t_frame = [e[0].elapsed_ms(e[3]) for e in evs]
+
+    ms_per_step = wall / args.steps * 1e3
+    value = N_OBJECTS / (wall / args.steps)
+    late_ms = statistics.mean(t_late)
+    late_bytes = N_OBJECTS * LATE_BYTES_PER_OBJECT + n_late * CMD_BYTES + 4
+    early_bytes = N_OBJECTS * EARLY_BYTES_PER_OBJECT + int(vis0.sum()) * 40 + n_early * CMD_BYTES + 4
+    pyr_bytes = DEPTH_W * DEPTH_H * 4 + sum(max(1, PYR_W >> L) * max(1, PYR_H >> L) for L in range(PYR_MIPS)) * 4
+    achieved = late_bytes / (late_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "kernel": "cull_kernel (late pass)", "algorithmic_bytes": late_bytes, "avg_us": round(late_ms * 1e3, 2)}
+
+    result = {
+        "metric": "object-culls/sec (4M objs, 1 view) + batched views/sec, p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "object-culls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "C1: 4,194,304 objects, 256 surfaces x 8 LODs, 1920x1080 depth -> 1024x512 "
+                               "11-mip pyramid, visibility Bernoulli(0.5), pyramid + early + late per step",
+                   "objects": N_OBJECTS, "copies_rotated": args.copies},
+        "passes_us": {"pyramid": round(statistics.mean(t_pyr) * 1e3, 2),
+                      "early": round(statistics.mean(t_early) * 1e3, 2),
+                      "late": round(late_ms * 1e3, 2),
+                      "frame_events": round(statistics.mean(t_frame) * 1e3, 2)},
+        "pass_gbs": {"pyramid": round(pyr_bytes / (statistics.mean(t_pyr) * 1e-3) / 1e9, 1),
+                     "early": round(early_bytes / (statistics.mean(t_early) * 1e-3) / 1e9, 1),
+                     "late": round(achieved, 1)},
+        "commands": {"early": n_early, "late": n_late},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline((sc.objects, sc.transforms, sc.surfaces, sc.lods), bytes(memoryview(v)),
+                                              depth, vis0, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,119 @@
+// bz_ctx.h — host-side state behind the C ABI: resident scenes, pyramids, per-stream scratch, graphs, server.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bz_internal.h"
+
+namespace bz {
+
+struct Scene {
+    uint32_t n_objects = 0, n_transforms = 0, n_surfaces = 0, n_lods = 0, n_clusters = 0;
+    bool xf_identity = false;
+    uint32_t* obj_surf = nullptr;
+    uint32_t* obj_xf = nullptr;
+    float4* xf_ps = nullptr;
+    float4* xf_q = nullptr;
+    float4* surf_sphere = nullptr;
+    uint2* surf_lod = nullptr;
+    float* lod_error = nullptr;
+    uint2* lod_draw = nullptr;
+    uint2* lod_cluster = nullptr;
+    uint32_t* vis = nullptr;
+    float4* cl_sphere = nullptr;
+    uint32_t* cl_cone = nullptr;
+    uint2* cl_draw = nullptr;
+    uint64_t bytes = 0;
+    // transform-update staging
+    void* upd_host = nullptr;
+    void* upd_dev = nullptr;
+    size_t upd_cap = 0;
+    hipEvent_t upd_event = nullptr;
+    SceneDev dev() const;
+    ~Scene();
+};
+
+struct Pyramid {
+    uint32_t base_w = 0, base_h = 0, mips = 0;
+    uint32_t level_offset[kMaxMips] = {};
+    uint64_t n_texels = 0;
+    float* texels = nullptr;
+    uint32_t* counter = nullptr;
+    unsigned long long* tail = nullptr;
+    uint32_t tile_w = 0, tile_h = 0, tiles_x = 0, tiles_y = 0, tile_levels = 0;
+    bool single_pass = false;
+    PyramidDesc desc() const;
+    PyrBuildJob build_job(const float* depth, uint32_t w, uint32_t h) const;
+    ~Pyramid();
+};
+
+struct StreamScratch {
+    LookbackScratch lb{};
+    uint32_t* err = nullptr;          // [0] look-back gave up, [1] pyramid, [2..3] spare
+    CullJob* jobs_dev = nullptr;
+    CullJob* jobs_host = nullptr;     // pinned
+    uint32_t jobs_cap = 0;
+    PyrBuildJob* pjobs_dev = nullptr;
+    PyrBuildJob* pjobs_host = nullptr;
+    uint32_t pjobs_cap = 0;
+    hipEvent_t upload_done = nullptr; // last job-table upload from the pinned buffers
+    ~StreamScratch();
+};
+
+struct Graph {
+    hipGraphExec_t exec = nullptr;
+    LookbackScratch lb{};
+    CullJob* jobs_dev = nullptr;
+    PyrBuildJob* pjobs_dev = nullptr;
+    ~Graph();
+};
+
+struct Server;  // server.cpp
+
+struct Ctx {
+    int device = 0;
+    uint32_t n_cu = 0;
+    uint64_t hbm_bytes = 0;
+    char device_name[64] = {};
+    std::string last_error;
+    std::mutex err_mu;
+    uint64_t next_handle = 0;
+    std::unordered_map<uint64_t, std::unique_ptr<Scene>> scenes;
+    std::unordered_map<uint64_t, std::unique_ptr<Pyramid>> pyramids;
+    std::unordered_map<uint64_t, std::unique_ptr<Graph>> graphs;
+    std::unordered_map<uintptr_t, std::unique_ptr<StreamScratch>> scratch;
+    bz_tuning tune[2] = {};
+    std::unique_ptr<Server> server;
+
+    Ctx();
+    ~Ctx();
+
+    bz_status fail(bz_status s, const std::string& what);
+    bz_status hip_fail(hipError_t e, const char* what);
+    void free_later(void* p);
+    Scene* scene(bz_scene h);
+    Pyramid* pyramid(bz_pyramid h);
+    bz_status scratch_for(hipStream_t st, uint32_t tiles, StreamScratch** out);
+    bz_status make_job(uint32_t pass, uint32_t flags, bz_scene sh, bz_pyramid ph, const bz_CameraViewData& view,
+                       uint32_t begin, uint32_t count, const uint32_t* vis_in, uint32_t* vis_out, void* cmds,
+                       uint32_t* count_dev, uint32_t capacity, uint32_t block, uint32_t items, CullJob* job);
+    bz_status make_batch(const bz_view_job* jobs, uint32_t n_jobs, const bz_tuning& t, std::vector<CullJob>& out,
+                         uint32_t* total_tiles);
+    uint32_t batch_grid(const bz_tuning& t, uint32_t total_tiles) const;
+    bz_status upload_jobs(StreamScratch* sc, const CullJob* jobs, uint32_t n, hipStream_t st);
+    bz_status upload_pjobs(StreamScratch* sc, const PyrBuildJob* jobs, uint32_t n, hipStream_t st);
+    bz_status build_pyramid(Pyramid* p, const float* depth, uint32_t w, uint32_t h, hipStream_t st);
+};
+
+ViewConsts make_view_consts(const bz_CameraViewData& v);
+uint32_t tiles_for(uint32_t n, uint32_t block, uint32_t items);
+
+}  // namespace bz
+
+struct bz_ctx : public bz::Ctx {};

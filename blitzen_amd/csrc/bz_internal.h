@@ -1,0 +1,103 @@
+// bz_internal.h — device-side job descriptors and kernel launchers shared by the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bzcull.h"
+#include "cull_math.h"
+
+namespace bz {
+
+// Resident scene in HBM, structure-of-arrays. Transforms are split into two 16-byte streams so every
+// per-object load of a wavefront is one fully coalesced 1 KiB access.
+struct SceneDev {
+    const uint32_t* obj_surf;   // [n_objects] RenderObject.surfaceId
+    const uint32_t* obj_xf;     // [n_objects] RenderObject.transformId; nullptr when transformId == objectId
+    const float4* xf_ps;        // [n_transforms] pos.xyz, scale
+    const float4* xf_q;         // [n_transforms] orientation
+    const float4* surf_sphere;  // [n_surfaces] center.xyz, radius
+    const uint2* surf_lod;      // [n_surfaces] lodOffset, lodCount
+    const float* lod_error;     // [n_lods]
+    const uint2* lod_draw;      // [n_lods] indexCount, firstIndex
+    const uint2* lod_cluster;   // [n_lods] clusterOffset, clusterCount
+    uint32_t n_objects, n_transforms, n_surfaces, n_lods;
+};
+
+enum : uint32_t { PASS_EARLY = 0, PASS_LATE = 1, PASS_PLAIN = 2, PASS_TEMPORAL = 3, PASS_ONPC = 4 };
+
+// One view's cull: a kernel argument for the single-view kernel and one entry of the batched job table.
+struct CullJob {
+    SceneDev scene;
+    ViewConsts view;
+    PyramidDesc pyr;              // pyr.texels == nullptr: no Hi-Z test
+    const uint32_t* vis_in;       // EARLY / LATE
+    uint32_t* vis_out;            // LATE
+    uint32_t* cmds;               // IndirectDrawData[capacity] as 6 dwords each
+    uint32_t* count;
+    uint32_t begin, n, capacity, pass;
+    uint32_t hiz;                 // apply the Hi-Z test
+    uint32_t tile_start;          // batched: first global tile of this job
+    uint32_t n_tiles;             // tiles of this job
+    uint32_t pad;
+};
+
+// Scratch for the single-pass decoupled look-back compaction. Self-resetting: the last workgroup to
+// finish clears status[], ticket and done, so back-to-back launches and graph replays need no memset.
+struct LookbackScratch {
+    unsigned long long* status;  // [capacity_tiles]
+    uint32_t* ticket;
+    uint32_t* done;
+    uint32_t* error;             // set to 1 when a bounded wait gives up
+    uint32_t capacity_tiles;
+};
+
+struct PyrBuildJob {
+    const float* depth;
+    uint32_t w, h;
+    float* texels;
+    uint32_t base_w, base_h, mips;
+    uint32_t tile_w, tile_h, tiles_x, tiles_y;  // single-pass tiling of level 0
+    uint32_t tile_levels;                        // levels reduced inside a tile (>= 1: includes level 0)
+    uint32_t wg_start;                           // batched: first workgroup of this job
+    uint32_t* counter;                           // per job: workgroups finished (self-resetting)
+    unsigned long long* tail;                    // per job: one flagged word per tile (self-resetting)
+    uint32_t level_offset[16];
+};
+
+constexpr int kMaxMips = 16;
+
+// Launchers (cull.hip). Each returns hipSuccess or the launch error.
+hipError_t launch_cull(const CullJob& job, const LookbackScratch& s, uint32_t block, uint32_t items, hipStream_t st);
+hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const LookbackScratch& s,
+                             uint32_t block, uint32_t items, uint32_t grid, hipStream_t st);
+hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st);
+hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, uint32_t n_jobs, uint32_t total_wgs, uint32_t max_mips,
+                                uint32_t* err, hipStream_t st);
+hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
+                                hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t st);
+hipError_t launch_scatter_transforms(const uint32_t* ids, const float4* src_ps, const float4* src_q, uint32_t n,
+                                     float4* xf_ps, float4* xf_q, hipStream_t st);
+hipError_t launch_synth_depth(uint64_t seed, uint32_t w, uint32_t h, uint32_t occluders, float p11, float znear,
+                              float* depth, hipStream_t st);
+
+// Cluster two-level cull
+struct ClusterJob {
+    SceneDev scene;
+    ViewConsts view;
+    PyramidDesc pyr;
+    const float4* cl_sphere;    // [n_clusters] center, radius
+    const uint32_t* cl_cone;    // [n_clusters] int8 x, y, z, cutoff packed
+    const uint2* cl_draw;       // [n_clusters] indexCount (= triangleCount*3), firstIndex (= dataOffset)
+    uint32_t* dispatch;         // ClusterDispatchData[dispatch_capacity] (3 dwords each)
+    uint32_t* dispatch_count;
+    uint32_t* cmds;
+    uint32_t* count;
+    uint32_t dispatch_capacity, capacity;
+    uint32_t tests, hiz;
+    uint32_t* work_offsets;     // scratch: per object-tile exclusive offsets into the record list
+};
+hipError_t launch_cluster_cull(const ClusterJob& job, const LookbackScratch& s, uint32_t* expand_total_dev,
+                               hipStream_t st);
+
+}  // namespace bz

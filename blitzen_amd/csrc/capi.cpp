@@ -1,0 +1,1089 @@
+// capi.cpp — the C ABI of bzcull (declared in include/bzcull.h).
+//
+// Host-side replacement of the Vulkan renderer's cull plumbing: buffer residency (vulkanRendererSetup.cpp:313-363,
+// 365-647), per-pass count reset + dispatch (vulkanDraw.cpp:107-226), depth pyramid creation and generation
+// (vulkanResources.cpp:85-116, vulkanDraw.cpp:554-622) and the per-frame transform stream (vulkanDraw.cpp:46-60,
+// 816-820). Every entry point validates its arguments, maps HIP failures to status codes and never throws.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bz_ctx.h"
+
+using namespace bz;
+
+static_assert(sizeof(bz_RenderObject) == 8, "RenderObject layout");
+static_assert(sizeof(bz_MeshTransform) == 32, "MeshTransform layout");
+static_assert(sizeof(bz_PrimitiveSurface) == 32, "PrimitiveSurface layout");
+static_assert(sizeof(bz_LodData) == 32, "LodData layout");
+static_assert(sizeof(bz_Cluster) == 32, "Cluster layout");
+static_assert(sizeof(bz_IndirectDrawData) == 24, "IndirectDrawData layout");
+static_assert(sizeof(bz_ClusterDispatchData) == 12, "ClusterDispatchData layout");
+static_assert(sizeof(bz_CameraViewData) == 256, "CameraViewData layout");
+static_assert(offsetof(bz_CameraViewData, lodTarget) == 180, "CameraViewData.lodTarget offset");
+static_assert(offsetof(bz_CameraViewData, frustumRight) == 140, "CameraViewData.frustumRight offset");
+static_assert(offsetof(bz_Cluster, dataOffset) == 20, "Cluster.dataOffset offset");
+
+namespace bz {
+
+bz_status Ctx::fail(bz_status s, const std::string& what)
+{
+    std::lock_guard<std::mutex> g(err_mu);
+    last_error = what;
+    return s;
+}
+
+bz_status Ctx::hip_fail(hipError_t e, const char* what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();  // clear the sticky launch error
+    return fail(e == hipErrorOutOfMemory ? BZ_E_OOM : BZ_E_HIP, m);
+}
+
+void Ctx::free_later(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+Scene* Ctx::scene(bz_scene h)
+{
+    auto it = scenes.find(h);
+    return it == scenes.end() ? nullptr : it->second.get();
+}
+
+Pyramid* Ctx::pyramid(bz_pyramid h)
+{
+    auto it = pyramids.find(h);
+    return it == pyramids.end() ? nullptr : it->second.get();
+}
+
+Scene::~Scene()
+{
+    void* ps[] = { obj_surf, obj_xf, xf_ps, xf_q, surf_sphere, surf_lod, lod_error, lod_draw, lod_cluster, vis,
+                   cl_sphere, cl_cone, cl_draw };
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+}
+
+Pyramid::~Pyramid()
+{
+    if (texels) (void)hipFree(texels);
+    if (counter) (void)hipFree(counter);
+    if (tail) (void)hipFree(tail);
+}
+
+StreamScratch::~StreamScratch()
+{
+    void* ps[] = { lb.status, lb.ticket, jobs_dev, pjobs_dev, err };
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (jobs_host) (void)hipHostFree(jobs_host);
+    if (pjobs_host) (void)hipHostFree(pjobs_host);
+    if (upload_done) (void)hipEventDestroy(upload_done);
+}
+
+// Look-back state for `stream`, sized for at least `tiles` tiles. Growth synchronizes the stream first.
+bz_status Ctx::scratch_for(hipStream_t st, uint32_t tiles, StreamScratch** out)
+{
+    auto& slot = scratch[(uintptr_t)st];
+    if (!slot) slot.reset(new StreamScratch());
+    StreamScratch& s = *slot;
+    if (!s.err) {
+        hipError_t e = hipMalloc((void**)&s.err, 16);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(error flags)");
+        e = hipMemset(s.err, 0, 16);
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(error flags)");
+        e = hipEventCreateWithFlags(&s.upload_done, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    }
+    if (s.lb.capacity_tiles < tiles) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return fail(BZ_E_STATE, "look-back scratch must grow, which is not allowed during stream capture; "
+                                    "run the call once before capturing");
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        if (s.lb.status) (void)hipFree(s.lb.status);
+        if (s.lb.ticket) (void)hipFree(s.lb.ticket);
+        s.lb.status = nullptr;
+        s.lb.ticket = nullptr;
+        s.lb.capacity_tiles = 0;
+        const uint32_t cap = std::max<uint32_t>(tiles, 1u << 17);
+        e = hipMalloc((void**)&s.lb.status, (size_t)cap * sizeof(unsigned long long));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(look-back status)");
+        e = hipMemset(s.lb.status, 0, (size_t)cap * sizeof(unsigned long long));
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(look-back status)");
+        e = hipMalloc((void**)&s.lb.ticket, 64);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(look-back counters)");
+        e = hipMemset(s.lb.ticket, 0, 64);
+        if (e != hipSuccess) return hip_fail(e, "hipMemset(look-back counters)");
+        s.lb.done = s.lb.ticket + 1;
+        s.lb.error = s.err;
+        s.lb.capacity_tiles = cap;
+    }
+    *out = &s;
+    return BZ_OK;
+}
+
+ViewConsts make_view_consts(const bz_CameraViewData& v)
+{
+    ViewConsts c;
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 4; ++col) c.m[4 * r + col] = v.view[col * 4 + r];
+    c.fR = v.frustumRight;
+    c.fL = v.frustumLeft;
+    c.fT = v.frustumTop;
+    c.fB = v.frustumBottom;
+    c.p00 = v.proj0;
+    c.p11 = v.proj5;
+    c.znear = v.zNear;
+    c.zfar = v.zFar;
+    c.pyrW = v.pyramidWidth;
+    c.pyrH = v.pyramidHeight;
+    c.lodTarget = v.lodTarget;
+    c.pad = 0.0f;
+    return c;
+}
+
+SceneDev Scene::dev() const
+{
+    SceneDev d;
+    d.obj_surf = obj_surf;
+    d.obj_xf = obj_xf;
+    d.xf_ps = xf_ps;
+    d.xf_q = xf_q;
+    d.surf_sphere = surf_sphere;
+    d.surf_lod = surf_lod;
+    d.lod_error = lod_error;
+    d.lod_draw = lod_draw;
+    d.lod_cluster = lod_cluster;
+    d.n_objects = n_objects;
+    d.n_transforms = n_transforms;
+    d.n_surfaces = n_surfaces;
+    d.n_lods = n_lods;
+    return d;
+}
+
+PyramidDesc Pyramid::desc() const
+{
+    PyramidDesc p;
+    std::memset(&p, 0, sizeof p);
+    p.texels = texels;
+    p.base_w = base_w;
+    p.base_h = base_h;
+    p.mips = mips;
+    for (uint32_t i = 0; i < mips && i < (uint32_t)kMaxMips; ++i) p.level_offset[i] = level_offset[i];
+    return p;
+}
+
+uint32_t tiles_for(uint32_t n, uint32_t block, uint32_t items)
+{
+    const uint64_t per = (uint64_t)block * items;
+    const uint64_t t = (n + per - 1) / per;
+    return t ? (uint32_t)t : 1u;
+}
+
+// Validates one view's cull and fills a CullJob (without tile_start).
+bz_status Ctx::make_job(uint32_t pass, uint32_t flags, bz_scene sh, bz_pyramid ph, const bz_CameraViewData& view,
+                        uint32_t begin, uint32_t count, const uint32_t* vis_in, uint32_t* vis_out, void* cmds,
+                        uint32_t* count_dev, uint32_t capacity, uint32_t block, uint32_t items, CullJob* job)
+{
+    Scene* s = scene(sh);
+    if (!s) return fail(BZ_E_INVALID, "unknown scene handle");
+    if (pass > PASS_ONPC) return fail(BZ_E_INVALID, "unknown pass");
+    if (!count_dev) return fail(BZ_E_INVALID, "count_dev is NULL");
+    if (capacity && !cmds) return fail(BZ_E_INVALID, "cmds_dev is NULL with capacity > 0");
+    if ((uintptr_t)cmds % 8u) return fail(BZ_E_INVALID, "cmds_dev must be 8-byte aligned");
+    if (begin == 0 && count == 0) count = s->n_objects;
+    if ((uint64_t)begin + count > s->n_objects) return fail(BZ_E_INVALID, "object range exceeds the scene");
+    Pyramid* p = nullptr;
+    if (ph) {
+        p = pyramid(ph);
+        if (!p) return fail(BZ_E_INVALID, "unknown pyramid handle");
+    }
+    const bool hiz = (pass == PASS_LATE || pass == PASS_TEMPORAL || (pass == PASS_EARLY && (flags & BZ_CULL_EARLY_HIZ)));
+    if (pass == PASS_LATE && !p) return fail(BZ_E_INVALID, "the LATE pass needs a pyramid");
+    if (pass == PASS_EARLY && (flags & BZ_CULL_EARLY_HIZ) && !p)
+        return fail(BZ_E_INVALID, "BZ_CULL_EARLY_HIZ needs a pyramid");
+    std::memset(job, 0, sizeof *job);
+    job->scene = s->dev();
+    job->view = make_view_consts(view);
+    if (hiz && p) job->pyr = p->desc();
+    job->hiz = (hiz && p) ? 1u : 0u;
+    job->vis_in = vis_in ? vis_in : s->vis;
+    job->vis_out = vis_out ? vis_out : const_cast<uint32_t*>(job->vis_in);
+    job->cmds = reinterpret_cast<uint32_t*>(cmds);
+    job->count = count_dev;
+    job->begin = begin;
+    job->n = count;
+    job->capacity = capacity;
+    job->pass = pass;
+    job->n_tiles = tiles_for(count, block, items);
+    return BZ_OK;
+}
+
+}  // namespace bz
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+extern "C" {
+
+uint32_t bz_abi_version(void) { return BZCULL_ABI_VERSION; }
+
+const char* bz_status_str(bz_status s)
+{
+    switch (s) {
+    case BZ_OK: return "ok";
+    case BZ_E_INVALID: return "invalid argument";
+    case BZ_E_OOM: return "out of memory";
+    case BZ_E_HIP: return "HIP runtime error";
+    case BZ_E_CAPACITY: return "output capacity exceeded";
+    case BZ_E_BUDGET: return "HBM budget exceeded";
+    case BZ_E_TIMEOUT: return "timeout";
+    case BZ_E_NOT_READY: return "not ready";
+    case BZ_E_NOT_FOUND: return "not found";
+    case BZ_E_STATE: return "invalid state";
+    case BZ_E_NODEVICE: return "no device";
+    case BZ_E_INTERNAL: return "internal kernel error";
+    case BZ_E_CANCELLED: return "cancelled";
+    default: return "unknown status";
+    }
+}
+
+const char* bz_last_error(const bz_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+bz_status bz_create(const bz_config* cfg, bz_ctx** out)
+{
+    if (!out) return BZ_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return BZ_E_NODEVICE;
+    }
+    const int dev = cfg ? cfg->device : 0;
+    if (dev < 0 || dev >= ndev) return BZ_E_INVALID;
+    if (hipSetDevice(dev) != hipSuccess) return BZ_E_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BZ_E_HIP;
+    std::unique_ptr<bz_ctx> c(new (std::nothrow) bz_ctx());
+    if (!c) return BZ_E_OOM;
+    c->device = dev;
+    c->n_cu = (uint32_t)prop.multiProcessorCount;
+    c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
+    std::snprintf(c->device_name, sizeof c->device_name, "%s", prop.gcnArchName);
+    c->tune[0] = bz_tuning{ 256, 4, 0, 0 };
+    c->tune[1] = bz_tuning{ 256, 4, 8, 0 };
+    *out = c.release();
+    return BZ_OK;
+}
+
+void bz_destroy(bz_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    bz_server_stop(ctx);
+    (void)hipDeviceSynchronize();
+    delete ctx;
+}
+
+bz_status bz_device_info(bz_ctx* ctx, char* name, uint32_t name_len, uint32_t* n_cu, uint64_t* hbm_bytes)
+{
+    if (!ctx) return BZ_E_INVALID;
+    if (name && name_len) std::snprintf(name, name_len, "%s", ctx->device_name);
+    if (n_cu) *n_cu = ctx->n_cu;
+    if (hbm_bytes) *hbm_bytes = ctx->hbm_bytes;
+    return BZ_OK;
+}
+
+// ---------------------------------------------------------------------------- scenes
+bz_status bz_scene_create(bz_ctx* ctx, const bz_RenderObject* objects, uint32_t n_objects,
+                          const bz_MeshTransform* transforms, uint32_t n_transforms,
+                          const bz_PrimitiveSurface* surfaces, uint32_t n_surfaces, const bz_LodData* lods,
+                          uint32_t n_lods, bz_scene* out)
+{
+    if (!ctx || !out) return BZ_E_INVALID;
+    *out = 0;
+    if ((n_objects && !objects) || (n_transforms && !transforms) || (n_surfaces && !surfaces) || (n_lods && !lods))
+        return ctx->fail(BZ_E_INVALID, "NULL array with non-zero count");
+    if (n_surfaces == 0 || n_lods == 0 || n_transforms == 0)
+        return ctx->fail(BZ_E_INVALID, "a scene needs at least one transform, surface and LOD");
+    // Validation: the kernels never bounds-check ids, so every id is checked here once.
+    for (uint32_t s = 0; s < n_surfaces; ++s) {
+        const bz_PrimitiveSurface& sf = surfaces[s];
+        if (sf.lodOffset >= n_lods || (uint64_t)sf.lodOffset + sf.lodCount > n_lods)
+            return ctx->fail(BZ_E_INVALID, "surface " + std::to_string(s) + " LOD range exceeds the LOD table");
+    }
+    bool identity = n_transforms >= n_objects;
+    for (uint32_t i = 0; i < n_objects; ++i) {
+        if (objects[i].transformId >= n_transforms)
+            return ctx->fail(BZ_E_INVALID, "object " + std::to_string(i) + " transformId out of range");
+        if (objects[i].surfaceId >= n_surfaces)
+            return ctx->fail(BZ_E_INVALID, "object " + std::to_string(i) + " surfaceId out of range");
+        identity = identity && objects[i].transformId == i;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(BZ_E_HIP, "hipSetDevice");
+    std::unique_ptr<Scene> s(new (std::nothrow) Scene());
+    if (!s) return BZ_E_OOM;
+    s->n_objects = n_objects;
+    s->n_transforms = n_transforms;
+    s->n_surfaces = n_surfaces;
+    s->n_lods = n_lods;
+    s->xf_identity = identity;
+    // host SoA staging
+    std::vector<uint32_t> surf_ids(n_objects), xf_ids(identity ? 0 : n_objects);
+    for (uint32_t i = 0; i < n_objects; ++i) {
+        surf_ids[i] = objects[i].surfaceId;
+        if (!identity) xf_ids[i] = objects[i].transformId;
+    }
+    std::vector<float4> ps(n_transforms), q(n_transforms), sph(n_surfaces);
+    std::vector<uint2> slod(n_surfaces), ldraw(n_lods), lcl(n_lods);
+    std::vector<float> lerr(n_lods);
+    for (uint32_t t = 0; t < n_transforms; ++t) {
+        const bz_MeshTransform& m = transforms[t];
+        ps[t] = make_float4(m.pos[0], m.pos[1], m.pos[2], m.scale);
+        q[t] = make_float4(m.orientation[0], m.orientation[1], m.orientation[2], m.orientation[3]);
+    }
+    for (uint32_t k = 0; k < n_surfaces; ++k) {
+        const bz_PrimitiveSurface& sf = surfaces[k];
+        sph[k] = make_float4(sf.center[0], sf.center[1], sf.center[2], sf.radius);
+        slod[k] = make_uint2(sf.lodOffset, sf.lodCount);
+    }
+    for (uint32_t l = 0; l < n_lods; ++l) {
+        lerr[l] = lods[l].error;
+        ldraw[l] = make_uint2(lods[l].indexCount, lods[l].firstIndex);
+        lcl[l] = make_uint2(lods[l].clusterOffset, lods[l].clusterCount);
+    }
+    auto up = [&](void** dst, const void* src, size_t bytes) -> bz_status {
+        if (bytes == 0) return BZ_OK;
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e != hipSuccess) return ctx->hip_fail(e, "hipMalloc(scene)");
+        s->bytes += bytes;
+        if (src) {
+            e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return ctx->hip_fail(e, "hipMemcpy(scene)");
+        }
+        return BZ_OK;
+    };
+    bz_status st;
+    if ((st = up((void**)&s->obj_surf, surf_ids.data(), surf_ids.size() * 4))) return st;
+    if (!identity && (st = up((void**)&s->obj_xf, xf_ids.data(), xf_ids.size() * 4))) return st;
+    if ((st = up((void**)&s->xf_ps, ps.data(), ps.size() * sizeof(float4)))) return st;
+    if ((st = up((void**)&s->xf_q, q.data(), q.size() * sizeof(float4)))) return st;
+    if ((st = up((void**)&s->surf_sphere, sph.data(), sph.size() * sizeof(float4)))) return st;
+    if ((st = up((void**)&s->surf_lod, slod.data(), slod.size() * sizeof(uint2)))) return st;
+    if ((st = up((void**)&s->lod_error, lerr.data(), lerr.size() * 4))) return st;
+    if ((st = up((void**)&s->lod_draw, ldraw.data(), ldraw.size() * sizeof(uint2)))) return st;
+    if ((st = up((void**)&s->lod_cluster, lcl.data(), lcl.size() * sizeof(uint2)))) return st;
+    if ((st = up((void**)&s->vis, nullptr, (size_t)std::max<uint32_t>(n_objects, 1u) * 4))) return st;
+    // visibility starts at zero (vulkanRendererSetup.cpp:349)
+    hipError_t e = launch_fill_u32(s->vis, 0u, std::max<uint32_t>(n_objects, 1u), nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return ctx->hip_fail(e, "visibility fill");
+    const bz_scene h = ++ctx->next_handle;
+    ctx->scenes[h] = std::move(s);
+    *out = h;
+    return BZ_OK;
+}
+
+bz_status bz_scene_set_clusters(bz_ctx* ctx, bz_scene scene, const bz_Cluster* clusters, uint32_t n_clusters)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (n_clusters && !clusters) return ctx->fail(BZ_E_INVALID, "NULL clusters");
+    // every LOD's cluster range must be inside the table
+    std::vector<uint2> lcl(s->n_lods);
+    if (hipMemcpy(lcl.data(), s->lod_cluster, lcl.size() * sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
+        return ctx->fail(BZ_E_HIP, "hipMemcpy(lod clusters)");
+    for (uint32_t l = 0; l < s->n_lods; ++l)
+        if (lcl[l].y && (uint64_t)lcl[l].x + lcl[l].y > n_clusters)
+            return ctx->fail(BZ_E_INVALID, "LOD " + std::to_string(l) + " cluster range exceeds the cluster table");
+    std::vector<float4> sph(n_clusters);
+    std::vector<uint32_t> cone(n_clusters);
+    std::vector<uint2> draw(n_clusters);
+    for (uint32_t c = 0; c < n_clusters; ++c) {
+        const bz_Cluster& k = clusters[c];
+        sph[c] = make_float4(k.center[0], k.center[1], k.center[2], k.radius);
+        cone[c] = (uint32_t)(uint8_t)k.coneAxisX | ((uint32_t)(uint8_t)k.coneAxisY << 8) |
+                  ((uint32_t)(uint8_t)k.coneAxisZ << 16) | ((uint32_t)(uint8_t)k.coneCutoff << 24);
+        draw[c] = make_uint2((uint32_t)k.triangleCount * 3u, k.dataOffset);
+    }
+    for (void* p : { (void*)s->cl_sphere, (void*)s->cl_cone, (void*)s->cl_draw })
+        if (p) (void)hipFree(p);
+    s->cl_sphere = nullptr;
+    s->cl_cone = nullptr;
+    s->cl_draw = nullptr;
+    s->n_clusters = n_clusters;
+    if (!n_clusters) return BZ_OK;
+    hipError_t e = hipMalloc((void**)&s->cl_sphere, sph.size() * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc((void**)&s->cl_cone, cone.size() * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->cl_draw, draw.size() * sizeof(uint2));
+    if (e == hipSuccess) e = hipMemcpy(s->cl_sphere, sph.data(), sph.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->cl_cone, cone.data(), cone.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->cl_draw, draw.data(), draw.size() * sizeof(uint2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return ctx->hip_fail(e, "cluster upload");
+    return BZ_OK;
+}
+
+bz_status bz_scene_destroy(bz_ctx* ctx, bz_scene scene)
+{
+    if (!ctx) return BZ_E_INVALID;
+    auto it = ctx->scenes.find(scene);
+    if (it == ctx->scenes.end()) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    (void)hipDeviceSynchronize();
+    ctx->scenes.erase(it);
+    return BZ_OK;
+}
+
+bz_status bz_scene_info(bz_ctx* ctx, bz_scene scene, uint32_t* n_objects, uint32_t* n_transforms, uint32_t* n_surfaces,
+                        uint32_t* n_lods, uint64_t* device_bytes)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (n_objects) *n_objects = s->n_objects;
+    if (n_transforms) *n_transforms = s->n_transforms;
+    if (n_surfaces) *n_surfaces = s->n_surfaces;
+    if (n_lods) *n_lods = s->n_lods;
+    if (device_bytes) *device_bytes = s->bytes;
+    return BZ_OK;
+}
+
+bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const uint32_t* ids, const bz_MeshTransform* xf,
+                                     uint32_t n, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (n == 0) return BZ_OK;
+    if (!ids || !xf) return ctx->fail(BZ_E_INVALID, "NULL ids / transforms");
+    for (uint32_t k = 0; k < n; ++k)
+        if (ids[k] >= s->n_transforms) return ctx->fail(BZ_E_INVALID, "transform id out of range");
+    hipStream_t st = (hipStream_t)stream;
+    // staging: one pinned block [ids | ps | q], reused once the previous upload on this scene completed
+    const size_t need = (size_t)n * (4 + 32);
+    if (s->upd_cap < need) {
+        if (s->upd_event) (void)hipEventSynchronize(s->upd_event);
+        if (s->upd_host) (void)hipHostFree(s->upd_host);
+        if (s->upd_dev) (void)hipFree(s->upd_dev);
+        s->upd_host = nullptr;
+        s->upd_dev = nullptr;
+        s->upd_cap = 0;
+        hipError_t e = hipHostMalloc((void**)&s->upd_host, need, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&s->upd_dev, need);
+        if (e == hipSuccess && !s->upd_event) e = hipEventCreateWithFlags(&s->upd_event, hipEventDisableTiming);
+        if (e != hipSuccess) return ctx->hip_fail(e, "transform staging");
+        s->upd_cap = need;
+    } else if (s->upd_event) {
+        (void)hipEventSynchronize(s->upd_event);
+    }
+    uint8_t* h = (uint8_t*)s->upd_host;
+    std::memcpy(h, ids, (size_t)n * 4);
+    const size_t ps_off = ((size_t)n * 4 + 15) / 16 * 16;
+    float4* hps = (float4*)(h + ps_off);
+    float4* hq = hps + n;
+    if (ps_off + (size_t)n * 32 > s->upd_cap) {
+        // alignment padding needs a little more room
+        (void)hipEventSynchronize(s->upd_event);
+        (void)hipHostFree(s->upd_host);
+        (void)hipFree(s->upd_dev);
+        const size_t need2 = ps_off + (size_t)n * 32;
+        hipError_t e = hipHostMalloc((void**)&s->upd_host, need2, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&s->upd_dev, need2);
+        if (e != hipSuccess) return ctx->hip_fail(e, "transform staging");
+        s->upd_cap = need2;
+        h = (uint8_t*)s->upd_host;
+        std::memcpy(h, ids, (size_t)n * 4);
+        hps = (float4*)(h + ps_off);
+        hq = hps + n;
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        hps[k] = make_float4(xf[k].pos[0], xf[k].pos[1], xf[k].pos[2], xf[k].scale);
+        hq[k] = make_float4(xf[k].orientation[0], xf[k].orientation[1], xf[k].orientation[2], xf[k].orientation[3]);
+    }
+    uint8_t* d = (uint8_t*)s->upd_dev;
+    hipError_t e = hipMemcpyAsync(d, h, ps_off + (size_t)n * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_scatter_transforms((const uint32_t*)d, (const float4*)(d + ps_off), (const float4*)(d + ps_off) + n, n,
+                                      s->xf_ps, s->xf_q, st);
+    if (e == hipSuccess) e = hipEventRecord(s->upd_event, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "transform update");
+    return BZ_OK;
+}
+
+bz_status bz_scene_set_visibility(bz_ctx* ctx, bz_scene scene, const uint32_t* vis, uint32_t n, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (n != s->n_objects || (n && !vis)) return ctx->fail(BZ_E_INVALID, "visibility size must equal n_objects");
+    if (!n) return BZ_OK;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(s->vis, vis, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "visibility upload");
+    return BZ_OK;
+}
+
+bz_status bz_scene_get_visibility(bz_ctx* ctx, bz_scene scene, uint32_t* vis, uint32_t n, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (n != s->n_objects || (n && !vis)) return ctx->fail(BZ_E_INVALID, "visibility size must equal n_objects");
+    if (!n) return BZ_OK;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(vis, s->vis, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "visibility download");
+    return BZ_OK;
+}
+
+bz_status bz_scene_visibility_ptr(bz_ctx* ctx, bz_scene scene, uint32_t** vis_dev)
+{
+    if (!ctx || !vis_dev) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    *vis_dev = s->vis;
+    return BZ_OK;
+}
+
+// ---------------------------------------------------------------------------- pyramids
+void bz_pyramid_reference_extent(uint32_t w, uint32_t h, uint32_t* base_w, uint32_t* base_h, uint32_t* mips)
+{
+    auto prev_pow2 = [](uint32_t v) {
+        uint32_t r = 1;
+        while (r * 2 < v) r *= 2;
+        return r;
+    };
+    const uint32_t bw = prev_pow2(w), bh = prev_pow2(h);
+    uint32_t n = 0, x = bw, y = bh;
+    while (x > 1 || y > 1) {
+        ++n;
+        x /= 2;
+        y /= 2;
+    }
+    if (base_w) *base_w = bw;
+    if (base_h) *base_h = bh;
+    if (mips) *mips = n;
+}
+
+bz_status bz_pyramid_create(bz_ctx* ctx, uint32_t base_w, uint32_t base_h, uint32_t mips, bz_pyramid* out)
+{
+    if (!ctx || !out) return BZ_E_INVALID;
+    *out = 0;
+    if (base_w == 0 || base_h == 0 || mips == 0 || mips > (uint32_t)kMaxMips)
+        return ctx->fail(BZ_E_INVALID, "pyramid extent must be non-zero with 1..16 mips");
+    std::unique_ptr<Pyramid> p(new (std::nothrow) Pyramid());
+    if (!p) return BZ_E_OOM;
+    p->base_w = base_w;
+    p->base_h = base_h;
+    p->mips = mips;
+    uint64_t off = 0;
+    for (uint32_t L = 0; L < mips; ++L) {
+        p->level_offset[L] = (uint32_t)off;
+        off += (uint64_t)level_dim(base_w, L) * level_dim(base_h, L);
+    }
+    if (off > 0xffffffffull) return ctx->fail(BZ_E_INVALID, "pyramid too large");
+    p->n_texels = off;
+    const bool pow2 = (base_w & (base_w - 1)) == 0 && (base_h & (base_h - 1)) == 0;
+    p->tile_w = std::min<uint32_t>(32u, base_w);
+    p->tile_h = std::min<uint32_t>(32u, base_h);
+    p->tiles_x = base_w / p->tile_w;
+    p->tiles_y = base_h / p->tile_h;
+    uint32_t tl = 1, m = std::max(p->tile_w, p->tile_h);
+    while (m > 1) {
+        m >>= 1;
+        ++tl;
+    }
+    p->tile_levels = std::min(tl, mips);
+    p->single_pass = pow2 && (p->tile_levels >= mips || (uint64_t)p->tiles_x * p->tiles_y <= 4096u);
+    hipError_t e = hipMalloc((void**)&p->texels, off * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->counter, 64);
+    if (e == hipSuccess) e = hipMemset(p->counter, 0, 64);
+    if (e == hipSuccess) e = hipMemset(p->texels, 0, off * 4);
+    const size_t n_tiles = (size_t)p->tiles_x * p->tiles_y;
+    if (e == hipSuccess) e = hipMalloc((void**)&p->tail, n_tiles * 8);
+    if (e == hipSuccess) e = hipMemset(p->tail, 0, n_tiles * 8);
+    if (e != hipSuccess) return ctx->hip_fail(e, "pyramid allocation");
+    const bz_pyramid h = ++ctx->next_handle;
+    ctx->pyramids[h] = std::move(p);
+    *out = h;
+    return BZ_OK;
+}
+
+bz_status bz_pyramid_destroy(bz_ctx* ctx, bz_pyramid pyramid)
+{
+    if (!ctx) return BZ_E_INVALID;
+    auto it = ctx->pyramids.find(pyramid);
+    if (it == ctx->pyramids.end()) return ctx->fail(BZ_E_INVALID, "unknown pyramid handle");
+    (void)hipDeviceSynchronize();
+    ctx->pyramids.erase(it);
+    return BZ_OK;
+}
+
+bz_status bz_pyramid_storage(bz_ctx* ctx, bz_pyramid pyramid, float** texels, uint64_t* n_texels, uint32_t* offsets)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Pyramid* p = ctx->pyramid(pyramid);
+    if (!p) return ctx->fail(BZ_E_INVALID, "unknown pyramid handle");
+    if (texels) *texels = p->texels;
+    if (n_texels) *n_texels = p->n_texels;
+    if (offsets)
+        for (uint32_t L = 0; L < p->mips; ++L) offsets[L] = p->level_offset[L];
+    return BZ_OK;
+}
+
+namespace bz {
+PyrBuildJob Pyramid::build_job(const float* depth, uint32_t w, uint32_t h) const
+{
+    PyrBuildJob j;
+    std::memset(&j, 0, sizeof j);
+    j.depth = depth;
+    j.w = w;
+    j.h = h;
+    j.texels = texels;
+    j.base_w = base_w;
+    j.base_h = base_h;
+    j.mips = mips;
+    j.tile_w = tile_w;
+    j.tile_h = tile_h;
+    j.tiles_x = tiles_x;
+    j.tiles_y = tiles_y;
+    j.tile_levels = tile_levels;
+    j.counter = counter;
+    j.tail = tail;
+    for (uint32_t L = 0; L < mips; ++L) j.level_offset[L] = level_offset[L];
+    return j;
+}
+
+bz_status Ctx::build_pyramid(Pyramid* p, const float* depth, uint32_t w, uint32_t h, hipStream_t st)
+{
+    StreamScratch* sc = nullptr;
+    bz_status s = scratch_for(st, 1, &sc);
+    if (s) return s;
+    hipError_t e = hipSuccess;
+    if (p->single_pass) {
+        e = launch_pyramid(p->build_job(depth, w, h), sc->err + 1, st);
+    } else {
+        e = launch_pyramid_level(depth, w, h, p->texels, p->base_w, p->base_h, st);
+        for (uint32_t L = 1; L < p->mips && e == hipSuccess; ++L)
+            e = launch_pyramid_level(p->texels + p->level_offset[L - 1], level_dim(p->base_w, L - 1),
+                                     level_dim(p->base_h, L - 1), p->texels + p->level_offset[L],
+                                     level_dim(p->base_w, L), level_dim(p->base_h, L), st);
+    }
+    if (e != hipSuccess) return hip_fail(e, "pyramid launch");
+    return BZ_OK;
+}
+}  // namespace bz
+
+bz_status bz_pyramid_build(bz_ctx* ctx, bz_pyramid pyramid, const float* depth, uint32_t w, uint32_t h, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Pyramid* p = ctx->pyramid(pyramid);
+    if (!p) return ctx->fail(BZ_E_INVALID, "unknown pyramid handle");
+    if (!depth || w == 0 || h == 0) return ctx->fail(BZ_E_INVALID, "empty depth buffer");
+    return ctx->build_pyramid(p, depth, w, h, (hipStream_t)stream);
+}
+
+bz_status bz_pyramid_build_batch(bz_ctx* ctx, const bz_pyramid_job* jobs, uint32_t n, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    if (n == 0) return BZ_OK;
+    if (!jobs) return ctx->fail(BZ_E_INVALID, "NULL jobs");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<PyrBuildJob> sp;
+    uint32_t wg = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        Pyramid* p = ctx->pyramid(jobs[k].pyramid);
+        if (!p) return ctx->fail(BZ_E_INVALID, "unknown pyramid handle");
+        if (!jobs[k].depth_dev || !jobs[k].w || !jobs[k].h) return ctx->fail(BZ_E_INVALID, "empty depth buffer");
+        if (!p->single_pass) {
+            bz_status s = ctx->build_pyramid(p, jobs[k].depth_dev, jobs[k].w, jobs[k].h, st);
+            if (s) return s;
+            continue;
+        }
+        PyrBuildJob j = p->build_job(jobs[k].depth_dev, jobs[k].w, jobs[k].h);
+        j.wg_start = wg;
+        wg += j.tiles_x * j.tiles_y;
+        sp.push_back(j);
+    }
+    if (sp.empty()) return BZ_OK;
+    StreamScratch* sc = nullptr;
+    bz_status s = ctx->scratch_for(st, 1, &sc);
+    if (s) return s;
+    s = ctx->upload_pjobs(sc, sp.data(), (uint32_t)sp.size(), st);
+    if (s) return s;
+    hipError_t e = launch_pyramid_batch(sc->pjobs_dev, (uint32_t)sp.size(), wg, 0, sc->err + 1, st);
+    if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "pyramid batch launch");
+    return BZ_OK;
+}
+
+// ---------------------------------------------------------------------------- cull
+bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
+{
+    if (!ctx || !a || !a->view) return BZ_E_INVALID;
+    const bz_tuning& t = ctx->tune[0];
+    CullJob job;
+    bz_status s = ctx->make_job(a->pass, a->flags, a->scene, a->pyramid, *a->view, a->object_begin, a->object_count,
+                                a->visibility_in_dev, a->visibility_out_dev, a->cmds_dev, a->count_dev, a->capacity,
+                                t.block, t.items, &job);
+    if (s) return s;
+    hipStream_t st = (hipStream_t)stream;
+    StreamScratch* sc = nullptr;
+    s = ctx->scratch_for(st, job.n_tiles, &sc);
+    if (s) return s;
+    hipError_t e = launch_cull(job, sc->lb, t.block, t.items, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "cull launch");
+    return BZ_OK;
+}
+
+namespace bz {
+bz_status Ctx::upload_jobs(StreamScratch* sc, const CullJob* jobs, uint32_t n, hipStream_t st)
+{
+    if (sc->jobs_cap < n) {
+        (void)hipEventSynchronize(sc->upload_done);
+        if (sc->jobs_dev) (void)hipFree(sc->jobs_dev);
+        if (sc->jobs_host) (void)hipHostFree(sc->jobs_host);
+        sc->jobs_dev = nullptr;
+        sc->jobs_host = nullptr;
+        sc->jobs_cap = 0;
+        const uint32_t cap = std::max<uint32_t>(n, 256u);
+        hipError_t e = hipMalloc((void**)&sc->jobs_dev, (size_t)cap * sizeof(CullJob));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sc->jobs_host, (size_t)cap * sizeof(CullJob), hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "job table allocation");
+        sc->jobs_cap = cap;
+    } else {
+        (void)hipEventSynchronize(sc->upload_done);  // the previous upload from jobs_host has completed
+    }
+    std::memcpy(sc->jobs_host, jobs, (size_t)n * sizeof(CullJob));
+    hipError_t e = hipMemcpyAsync(sc->jobs_dev, sc->jobs_host, (size_t)n * sizeof(CullJob), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "job table upload");
+    return BZ_OK;
+}
+
+bz_status Ctx::upload_pjobs(StreamScratch* sc, const PyrBuildJob* jobs, uint32_t n, hipStream_t st)
+{
+    if (sc->pjobs_cap < n) {
+        (void)hipEventSynchronize(sc->upload_done);
+        if (sc->pjobs_dev) (void)hipFree(sc->pjobs_dev);
+        if (sc->pjobs_host) (void)hipHostFree(sc->pjobs_host);
+        sc->pjobs_dev = nullptr;
+        sc->pjobs_host = nullptr;
+        sc->pjobs_cap = 0;
+        const uint32_t cap = std::max<uint32_t>(n, 256u);
+        hipError_t e = hipMalloc((void**)&sc->pjobs_dev, (size_t)cap * sizeof(PyrBuildJob));
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&sc->pjobs_host, (size_t)cap * sizeof(PyrBuildJob), hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "pyramid job table allocation");
+        sc->pjobs_cap = cap;
+    } else {
+        (void)hipEventSynchronize(sc->upload_done);
+    }
+    std::memcpy(sc->pjobs_host, jobs, (size_t)n * sizeof(PyrBuildJob));
+    hipError_t e =
+        hipMemcpyAsync(sc->pjobs_dev, sc->pjobs_host, (size_t)n * sizeof(PyrBuildJob), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "pyramid job table upload");
+    return BZ_OK;
+}
+
+bz_status Ctx::make_batch(const bz_view_job* jobs, uint32_t n_jobs, const bz_tuning& t, std::vector<CullJob>& out,
+                          uint32_t* total_tiles)
+{
+    out.resize(n_jobs);
+    uint64_t tiles = 0;
+    for (uint32_t k = 0; k < n_jobs; ++k) {
+        const bz_view_job& v = jobs[k];
+        if (v.pass != PASS_LATE && v.pass != PASS_PLAIN && v.pass != PASS_TEMPORAL)
+            return fail(BZ_E_INVALID, "batched jobs support the LATE, PLAIN and TEMPORAL passes");
+        bz_status s = make_job(v.pass, 0, v.scene, v.pyramid, v.view, v.object_begin, v.object_count,
+                               v.visibility_in_dev, v.visibility_out_dev, v.cmds_dev, v.count_dev, v.capacity, t.block,
+                               t.items, &out[k]);
+        if (s) return s;
+        out[k].tile_start = (uint32_t)tiles;
+        tiles += out[k].n_tiles;
+        if (tiles > 0x7fffffffull) return fail(BZ_E_INVALID, "batch too large");
+    }
+    *total_tiles = (uint32_t)tiles;
+    return BZ_OK;
+}
+
+uint32_t Ctx::batch_grid(const bz_tuning& t, uint32_t total_tiles) const
+{
+    uint64_t g = (uint64_t)n_cu * std::max<uint32_t>(t.grid_per_cu, 1u);
+    if (g > total_tiles) g = total_tiles;
+    return g ? (uint32_t)g : 1u;
+}
+}  // namespace bz
+
+bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    if (n_jobs == 0) return BZ_OK;
+    if (!jobs) return ctx->fail(BZ_E_INVALID, "NULL jobs");
+    const bz_tuning t = ctx->tune[1];
+    std::vector<CullJob> cj;
+    uint32_t tiles = 0;
+    bz_status s = ctx->make_batch(jobs, n_jobs, t, cj, &tiles);
+    if (s) return s;
+    hipStream_t st = (hipStream_t)stream;
+    StreamScratch* sc = nullptr;
+    s = ctx->scratch_for(st, tiles, &sc);
+    if (s) return s;
+    s = ctx->upload_jobs(sc, cj.data(), n_jobs, st);
+    if (s) return s;
+    hipError_t e = launch_cull_batch(sc->jobs_dev, n_jobs, tiles, sc->lb, t.block, t.items, ctx->batch_grid(t, tiles), st);
+    if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "batched cull launch");
+    return BZ_OK;
+}
+
+bz_status bz_get_tuning(bz_ctx* ctx, uint32_t kernel, bz_tuning* out)
+{
+    if (!ctx || !out || kernel > 1) return BZ_E_INVALID;
+    *out = ctx->tune[kernel];
+    return BZ_OK;
+}
+
+bz_status bz_set_tuning(bz_ctx* ctx, uint32_t kernel, const bz_tuning* t)
+{
+    if (!ctx || !t || kernel > 1) return BZ_E_INVALID;
+    const bool blk = t->block == 64 || t->block == 128 || t->block == 256 || t->block == 512;
+    const bool it = t->items == 1 || t->items == 2 || t->items == 4 || t->items == 8;
+    if (!blk || !it) return ctx->fail(BZ_E_INVALID, "block must be 64/128/256/512 and items 1/2/4/8");
+    ctx->tune[kernel] = *t;
+    if (ctx->tune[kernel].grid_per_cu == 0) ctx->tune[kernel].grid_per_cu = 8;
+    return BZ_OK;
+}
+
+bz_status bz_check(bz_ctx* ctx, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    auto it = ctx->scratch.find((uintptr_t)st);
+    if (it == ctx->scratch.end() || !it->second->err) return BZ_OK;
+    uint32_t flags[4] = { 0, 0, 0, 0 };
+    e = hipMemcpy(flags, it->second->err, sizeof flags, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return ctx->hip_fail(e, "error flag read");
+    if (flags[0] || flags[1]) {
+        (void)hipMemset(it->second->err, 0, 16);
+        return ctx->fail(BZ_E_INTERNAL, "a kernel gave up a bounded wait");
+    }
+    return BZ_OK;
+}
+
+// ---------------------------------------------------------------------------- graphs
+bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uint32_t n_pyr, const bz_view_job* jobs,
+                                uint32_t n_jobs, bz_graph* out)
+{
+    if (!ctx || !out) return BZ_E_INVALID;
+    *out = 0;
+    if (n_jobs && !jobs) return ctx->fail(BZ_E_INVALID, "NULL jobs");
+    if (n_pyr && !pyr_jobs) return ctx->fail(BZ_E_INVALID, "NULL pyramid jobs");
+    std::unique_ptr<Graph> g(new (std::nothrow) Graph());
+    if (!g) return BZ_E_OOM;
+    const bz_tuning t = ctx->tune[1];
+    std::vector<CullJob> cj;
+    uint32_t tiles = 0;
+    bz_status s = n_jobs ? ctx->make_batch(jobs, n_jobs, t, cj, &tiles) : BZ_OK;
+    if (s) return s;
+    std::vector<PyrBuildJob> pj;
+    std::vector<std::pair<Pyramid*, bz_pyramid_job>> multi;
+    uint32_t wg = 0;
+    for (uint32_t k = 0; k < n_pyr; ++k) {
+        Pyramid* p = ctx->pyramid(pyr_jobs[k].pyramid);
+        if (!p) return ctx->fail(BZ_E_INVALID, "unknown pyramid handle");
+        if (!p->single_pass) {
+            multi.push_back({ p, pyr_jobs[k] });
+            continue;
+        }
+        PyrBuildJob j = p->build_job(pyr_jobs[k].depth_dev, pyr_jobs[k].w, pyr_jobs[k].h);
+        j.wg_start = wg;
+        wg += j.tiles_x * j.tiles_y;
+        pj.push_back(j);
+    }
+    // graph-owned device state
+    hipError_t e = hipSuccess;
+    const uint32_t cap = std::max<uint32_t>(tiles, 1u);
+    e = hipMalloc((void**)&g->lb.status, (size_t)cap * 8);
+    if (e == hipSuccess) e = hipMemset(g->lb.status, 0, (size_t)cap * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&g->lb.ticket, 64);
+    if (e == hipSuccess) e = hipMemset(g->lb.ticket, 0, 64);
+    if (e != hipSuccess) return ctx->hip_fail(e, "graph scratch");
+    g->lb.done = g->lb.ticket + 1;
+    g->lb.error = g->lb.ticket + 2;
+    g->lb.capacity_tiles = cap;
+    if (n_jobs) {
+        e = hipMalloc((void**)&g->jobs_dev, cj.size() * sizeof(CullJob));
+        if (e == hipSuccess) e = hipMemcpy(g->jobs_dev, cj.data(), cj.size() * sizeof(CullJob), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !pj.empty()) {
+        e = hipMalloc((void**)&g->pjobs_dev, pj.size() * sizeof(PyrBuildJob));
+        if (e == hipSuccess)
+            e = hipMemcpy(g->pjobs_dev, pj.data(), pj.size() * sizeof(PyrBuildJob), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "graph job tables");
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return ctx->hip_fail(e, "capture stream");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess && !pj.empty())
+        e = launch_pyramid_batch(g->pjobs_dev, (uint32_t)pj.size(), wg, 0, g->lb.ticket + 3, cs);
+    for (auto& m : multi) {
+        if (e != hipSuccess) break;
+        Pyramid* p = m.first;
+        e = launch_pyramid_level(m.second.depth_dev, m.second.w, m.second.h, p->texels, p->base_w, p->base_h, cs);
+        for (uint32_t L = 1; L < p->mips && e == hipSuccess; ++L)
+            e = launch_pyramid_level(p->texels + p->level_offset[L - 1], level_dim(p->base_w, L - 1),
+                                     level_dim(p->base_h, L - 1), p->texels + p->level_offset[L],
+                                     level_dim(p->base_w, L), level_dim(p->base_h, L), cs);
+    }
+    if (e == hipSuccess && n_jobs)
+        e = launch_cull_batch(g->jobs_dev, n_jobs, tiles, g->lb, t.block, t.items, ctx->batch_grid(t, tiles), cs);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    (void)hipStreamDestroy(cs);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return ctx->hip_fail(e, "graph capture");
+    const bz_graph h = ++ctx->next_handle;
+    ctx->graphs[h] = std::move(g);
+    *out = h;
+    return BZ_OK;
+}
+
+bz_status bz_graph_launch(bz_ctx* ctx, bz_graph graph, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    auto it = ctx->graphs.find(graph);
+    if (it == ctx->graphs.end()) return ctx->fail(BZ_E_INVALID, "unknown graph handle");
+    hipError_t e = hipGraphLaunch(it->second->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipGraphLaunch");
+    return BZ_OK;
+}
+
+bz_status bz_graph_destroy(bz_ctx* ctx, bz_graph graph)
+{
+    if (!ctx) return BZ_E_INVALID;
+    auto it = ctx->graphs.find(graph);
+    if (it == ctx->graphs.end()) return ctx->fail(BZ_E_INVALID, "unknown graph handle");
+    (void)hipDeviceSynchronize();
+    ctx->graphs.erase(it);
+    return BZ_OK;
+}
+
+namespace bz {
+Graph::~Graph()
+{
+    if (exec) (void)hipGraphExecDestroy(exec);
+    void* ps[] = { lb.status, lb.ticket, jobs_dev, pjobs_dev };
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+}
+}  // namespace bz
+
+// ---------------------------------------------------------------------------- device plumbing
+bz_status bz_dev_alloc(bz_ctx* ctx, uint64_t bytes, void** out)
+{
+    if (!ctx || !out) return BZ_E_INVALID;
+    *out = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipMalloc");
+    return BZ_OK;
+}
+
+bz_status bz_dev_free(bz_ctx* ctx, void* p)
+{
+    if (!ctx) return BZ_E_INVALID;
+    if (!p) return BZ_OK;
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipFree");
+    return BZ_OK;
+}
+
+static bz_status copy(bz_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipMemcpyKind k, void* stream,
+                      bool sync)
+{
+    if (!ctx || (bytes && (!dst || !src))) return BZ_E_INVALID;
+    if (!bytes) return BZ_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream);
+    if (e == hipSuccess && sync) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipMemcpyAsync");
+    return BZ_OK;
+}
+
+bz_status bz_memcpy_h2d(bz_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return copy(ctx, dst, src, bytes, hipMemcpyHostToDevice, stream, true);
+}
+bz_status bz_memcpy_d2h(bz_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, stream, true);
+}
+bz_status bz_memcpy_d2d(bz_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return copy(ctx, dst, src, bytes, hipMemcpyDeviceToDevice, stream, false);
+}
+
+bz_status bz_memset_d32(bz_ctx* ctx, void* dst, uint32_t value, uint64_t count, void* stream)
+{
+    if (!ctx || (count && !dst)) return BZ_E_INVALID;
+    hipError_t e = launch_fill_u32((uint32_t*)dst, value, count, (hipStream_t)stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "fill");
+    return BZ_OK;
+}
+
+bz_status bz_stream_create(bz_ctx* ctx, void** stream)
+{
+    if (!ctx || !stream) return BZ_E_INVALID;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamCreate");
+    *stream = s;
+    return BZ_OK;
+}
+
+bz_status bz_stream_destroy(bz_ctx* ctx, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    if (!stream) return BZ_OK;
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    ctx->scratch.erase((uintptr_t)stream);
+    hipError_t e = hipStreamDestroy((hipStream_t)stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamDestroy");
+    return BZ_OK;
+}
+
+bz_status bz_stream_sync(bz_ctx* ctx, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    return BZ_OK;
+}
+
+bz_status bz_synth_depth_device(bz_ctx* ctx, uint64_t seed, uint32_t w, uint32_t h, uint32_t occluders, float p11,
+                                float z_near, float* depth_dev, void* stream)
+{
+    if (!ctx || !depth_dev || !w || !h) return BZ_E_INVALID;
+    if (occluders > 4096) return ctx->fail(BZ_E_INVALID, "at most 4096 occluders");
+    hipError_t e = launch_synth_depth(seed, w, h, occluders, p11, z_near, depth_dev, (hipStream_t)stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "synthetic depth launch");
+    return BZ_OK;
+}
+
+}  // extern "C"
