@@ -44,7 +44,6 @@ struct Pyramid {
     uint32_t level_offset[kMaxMips] = {};
     uint64_t n_texels = 0;
     float* texels = nullptr;
-    uint32_t* counter = nullptr;
     unsigned long long* tail = nullptr;
     uint32_t tile_w = 0, tile_h = 0, tiles_x = 0, tiles_y = 0, tile_levels = 0;
     bool single_pass = false;
@@ -54,8 +53,8 @@ struct Pyramid {
 };
 
 struct StreamScratch {
-    LookbackScratch lb{};
-    uint32_t* err = nullptr;          // [0] look-back gave up, [1] pyramid, [2..3] spare
+    CompactScratch cs{};
+    uint32_t* err = nullptr;          // [0] spare, [1] pyramid tail wait gave up, [2..3] spare
     CullJob* jobs_dev = nullptr;
     CullJob* jobs_host = nullptr;     // pinned
     uint32_t jobs_cap = 0;
@@ -68,7 +67,8 @@ struct StreamScratch {
 
 struct Graph {
     hipGraphExec_t exec = nullptr;
-    LookbackScratch lb{};
+    CompactScratch cs{};
+    uint32_t* err = nullptr;
     CullJob* jobs_dev = nullptr;
     PyrBuildJob* pjobs_dev = nullptr;
     ~Graph();
@@ -99,7 +99,7 @@ struct Ctx {
     void free_later(void* p);
     Scene* scene(bz_scene h);
     Pyramid* pyramid(bz_pyramid h);
-    bz_status scratch_for(hipStream_t st, uint32_t tiles, StreamScratch** out);
+    bz_status scratch_for(hipStream_t st, uint32_t tiles, uint64_t entries, StreamScratch** out);
     bz_status make_job(uint32_t pass, uint32_t flags, bz_scene sh, bz_pyramid ph, const bz_CameraViewData& view,
                        uint32_t begin, uint32_t count, const uint32_t* vis_in, uint32_t* vis_out, void* cmds,
                        uint32_t* count_dev, uint32_t capacity, uint32_t block, uint32_t items, CullJob* job);

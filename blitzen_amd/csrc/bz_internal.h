@@ -41,14 +41,13 @@ struct CullJob {
     uint32_t pad;
 };
 
-// Scratch for the single-pass decoupled look-back compaction. Self-resetting: the last workgroup to
-// finish clears status[], ticket and done, so back-to-back launches and graph replays need no memset.
-struct LookbackScratch {
-    unsigned long long* status;  // [capacity_tiles]
-    uint32_t* ticket;
-    uint32_t* done;
-    uint32_t* error;             // set to 1 when a bounded wait gives up
+// Scratch of the two-kernel compaction: per-tile command counts and per-tile staging slots (objectId, LOD) of
+// tile_size entries each. Fully rewritten by every decide launch; nothing needs clearing between launches.
+struct CompactScratch {
+    uint32_t* counts;            // [capacity_tiles]
+    uint2* staging;              // [capacity_entries]
     uint32_t capacity_tiles;
+    uint64_t capacity_entries;
 };
 
 struct PyrBuildJob {
@@ -59,17 +58,19 @@ struct PyrBuildJob {
     uint32_t tile_w, tile_h, tiles_x, tiles_y;  // single-pass tiling of level 0
     uint32_t tile_levels;                        // levels reduced inside a tile (>= 1: includes level 0)
     uint32_t wg_start;                           // batched: first workgroup of this job
-    uint32_t* counter;                           // per job: workgroups finished (self-resetting)
-    unsigned long long* tail;                    // per job: one flagged word per tile (self-resetting)
+    unsigned long long* tail;                    // per job: one flagged word per tile, zero between builds
     uint32_t level_offset[16];
 };
 
 constexpr int kMaxMips = 16;
 
 // Launchers (cull.hip). Each returns hipSuccess or the launch error.
-hipError_t launch_cull(const CullJob& job, const LookbackScratch& s, uint32_t block, uint32_t items, hipStream_t st);
-hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const LookbackScratch& s,
-                             uint32_t block, uint32_t items, uint32_t grid, hipStream_t st);
+// The decide kernel is persistent: n_cu * per_cu workgroups (per_cu == 0: as many as are resident per CU),
+// clamped to the tile count.
+hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
+                       uint32_t per_cu, hipStream_t st);
+hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const CompactScratch& s,
+                             uint32_t block, uint32_t items, hipStream_t st);
 hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st);
 hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, uint32_t n_jobs, uint32_t total_wgs, uint32_t max_mips,
                                 uint32_t* err, hipStream_t st);
@@ -97,7 +98,7 @@ struct ClusterJob {
     uint32_t tests, hiz;
     uint32_t* work_offsets;     // scratch: per object-tile exclusive offsets into the record list
 };
-hipError_t launch_cluster_cull(const ClusterJob& job, const LookbackScratch& s, uint32_t* expand_total_dev,
+hipError_t launch_cluster_cull(const ClusterJob& job, const CompactScratch& s, uint32_t* expand_total_dev,
                                hipStream_t st);
 
 }  // namespace bz

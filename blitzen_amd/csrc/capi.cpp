@@ -75,13 +75,12 @@ Scene::~Scene()
 Pyramid::~Pyramid()
 {
     if (texels) (void)hipFree(texels);
-    if (counter) (void)hipFree(counter);
     if (tail) (void)hipFree(tail);
 }
 
 StreamScratch::~StreamScratch()
 {
-    void* ps[] = { lb.status, lb.ticket, jobs_dev, pjobs_dev, err };
+    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err };
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (jobs_host) (void)hipHostFree(jobs_host);
@@ -89,8 +88,9 @@ StreamScratch::~StreamScratch()
     if (upload_done) (void)hipEventDestroy(upload_done);
 }
 
-// Look-back state for `stream`, sized for at least `tiles` tiles. Growth synchronizes the stream first.
-bz_status Ctx::scratch_for(hipStream_t st, uint32_t tiles, StreamScratch** out)
+// Compaction scratch for `stream`, sized for at least `tiles` tiles and `entries` staging entries. Growth
+// synchronizes the stream first and is refused while the stream is being captured into a graph.
+bz_status Ctx::scratch_for(hipStream_t st, uint32_t tiles, uint64_t entries, StreamScratch** out)
 {
     auto& slot = scratch[(uintptr_t)st];
     if (!slot) slot.reset(new StreamScratch());
@@ -103,30 +103,23 @@ bz_status Ctx::scratch_for(hipStream_t st, uint32_t tiles, StreamScratch** out)
         e = hipEventCreateWithFlags(&s.upload_done, hipEventDisableTiming);
         if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
     }
-    if (s.lb.capacity_tiles < tiles) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
-            return fail(BZ_E_STATE, "look-back scratch must grow, which is not allowed during stream capture; "
+    if (s.cs.capacity_tiles < tiles || s.cs.capacity_entries < entries) {
+        hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cst) == hipSuccess && cst != hipStreamCaptureStatusNone)
+            return fail(BZ_E_STATE, "compaction scratch must grow, which is not allowed during stream capture; "
                                     "run the call once before capturing");
         hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-        if (s.lb.status) (void)hipFree(s.lb.status);
-        if (s.lb.ticket) (void)hipFree(s.lb.ticket);
-        s.lb.status = nullptr;
-        s.lb.ticket = nullptr;
-        s.lb.capacity_tiles = 0;
-        const uint32_t cap = std::max<uint32_t>(tiles, 1u << 17);
-        e = hipMalloc((void**)&s.lb.status, (size_t)cap * sizeof(unsigned long long));
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc(look-back status)");
-        e = hipMemset(s.lb.status, 0, (size_t)cap * sizeof(unsigned long long));
-        if (e != hipSuccess) return hip_fail(e, "hipMemset(look-back status)");
-        e = hipMalloc((void**)&s.lb.ticket, 64);
-        if (e != hipSuccess) return hip_fail(e, "hipMalloc(look-back counters)");
-        e = hipMemset(s.lb.ticket, 0, 64);
-        if (e != hipSuccess) return hip_fail(e, "hipMemset(look-back counters)");
-        s.lb.done = s.lb.ticket + 1;
-        s.lb.error = s.err;
-        s.lb.capacity_tiles = cap;
+        const uint32_t cap_t = std::max<uint32_t>(std::max(tiles, s.cs.capacity_tiles), 1u << 12);
+        const uint64_t cap_e = std::max<uint64_t>(std::max(entries, s.cs.capacity_entries), 1ull << 20);
+        if (s.cs.counts) (void)hipFree(s.cs.counts);
+        if (s.cs.staging) (void)hipFree(s.cs.staging);
+        s.cs = CompactScratch{};
+        e = hipMalloc((void**)&s.cs.counts, (size_t)cap_t * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.cs.staging, (size_t)cap_e * sizeof(uint2));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(compaction scratch)");
+        s.cs.capacity_tiles = cap_t;
+        s.cs.capacity_entries = cap_e;
     }
     *out = &s;
     return BZ_OK;
@@ -280,7 +273,7 @@ bz_status bz_create(const bz_config* cfg, bz_ctx** out)
     c->n_cu = (uint32_t)prop.multiProcessorCount;
     c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
     std::snprintf(c->device_name, sizeof c->device_name, "%s", prop.gcnArchName);
-    c->tune[0] = bz_tuning{ 256, 4, 0, 0 };
+    c->tune[0] = bz_tuning{ 512, 2, 0, 0 };
     c->tune[1] = bz_tuning{ 256, 4, 8, 0 };
     *out = c.release();
     return BZ_OK;
@@ -608,8 +601,6 @@ bz_status bz_pyramid_create(bz_ctx* ctx, uint32_t base_w, uint32_t base_h, uint3
     p->tile_levels = std::min(tl, mips);
     p->single_pass = pow2 && (p->tile_levels >= mips || (uint64_t)p->tiles_x * p->tiles_y <= 4096u);
     hipError_t e = hipMalloc((void**)&p->texels, off * 4);
-    if (e == hipSuccess) e = hipMalloc((void**)&p->counter, 64);
-    if (e == hipSuccess) e = hipMemset(p->counter, 0, 64);
     if (e == hipSuccess) e = hipMemset(p->texels, 0, off * 4);
     const size_t n_tiles = (size_t)p->tiles_x * p->tiles_y;
     if (e == hipSuccess) e = hipMalloc((void**)&p->tail, n_tiles * 8);
@@ -660,7 +651,6 @@ PyrBuildJob Pyramid::build_job(const float* depth, uint32_t w, uint32_t h) const
     j.tiles_x = tiles_x;
     j.tiles_y = tiles_y;
     j.tile_levels = tile_levels;
-    j.counter = counter;
     j.tail = tail;
     for (uint32_t L = 0; L < mips; ++L) j.level_offset[L] = level_offset[L];
     return j;
@@ -669,7 +659,7 @@ PyrBuildJob Pyramid::build_job(const float* depth, uint32_t w, uint32_t h) const
 bz_status Ctx::build_pyramid(Pyramid* p, const float* depth, uint32_t w, uint32_t h, hipStream_t st)
 {
     StreamScratch* sc = nullptr;
-    bz_status s = scratch_for(st, 1, &sc);
+    bz_status s = scratch_for(st, 0, 0, &sc);
     if (s) return s;
     hipError_t e = hipSuccess;
     if (p->single_pass) {
@@ -718,8 +708,13 @@ bz_status bz_pyramid_build_batch(bz_ctx* ctx, const bz_pyramid_job* jobs, uint32
         sp.push_back(j);
     }
     if (sp.empty()) return BZ_OK;
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return ctx->fail(BZ_E_STATE, "bz_pyramid_build_batch cannot be stream-captured; use bz_batch_graph_create");
+    }
     StreamScratch* sc = nullptr;
-    bz_status s = ctx->scratch_for(st, 1, &sc);
+    bz_status s = ctx->scratch_for(st, 0, 0, &sc);
     if (s) return s;
     s = ctx->upload_pjobs(sc, sp.data(), (uint32_t)sp.size(), st);
     if (s) return s;
@@ -741,9 +736,14 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
     if (s) return s;
     hipStream_t st = (hipStream_t)stream;
     StreamScratch* sc = nullptr;
-    s = ctx->scratch_for(st, job.n_tiles, &sc);
+    if (job.n == 0) {  // nothing to cull: the count is 0 and the visibility buffers are untouched
+        hipError_t e = launch_fill_u32(job.count, 0u, 1, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "count fill");
+        return BZ_OK;
+    }
+    s = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * t.block * t.items, &sc);
     if (s) return s;
-    hipError_t e = launch_cull(job, sc->lb, t.block, t.items, st);
+    hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "cull launch");
     return BZ_OK;
 }
@@ -838,11 +838,16 @@ bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, v
     if (s) return s;
     hipStream_t st = (hipStream_t)stream;
     StreamScratch* sc = nullptr;
-    s = ctx->scratch_for(st, tiles, &sc);
+    s = ctx->scratch_for(st, tiles, (uint64_t)tiles * t.block * t.items, &sc);
     if (s) return s;
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return ctx->fail(BZ_E_STATE, "bz_cull_batch cannot be stream-captured; use bz_batch_graph_create");
+    }
     s = ctx->upload_jobs(sc, cj.data(), n_jobs, st);
     if (s) return s;
-    hipError_t e = launch_cull_batch(sc->jobs_dev, n_jobs, tiles, sc->lb, t.block, t.items, ctx->batch_grid(t, tiles), st);
+    hipError_t e = launch_cull_batch(sc->jobs_dev, n_jobs, tiles, sc->cs, t.block, t.items, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "batched cull launch");
     return BZ_OK;
@@ -858,11 +863,10 @@ bz_status bz_get_tuning(bz_ctx* ctx, uint32_t kernel, bz_tuning* out)
 bz_status bz_set_tuning(bz_ctx* ctx, uint32_t kernel, const bz_tuning* t)
 {
     if (!ctx || !t || kernel > 1) return BZ_E_INVALID;
-    const bool blk = t->block == 64 || t->block == 128 || t->block == 256 || t->block == 512;
-    const bool it = t->items == 1 || t->items == 2 || t->items == 4 || t->items == 8;
-    if (!blk || !it) return ctx->fail(BZ_E_INVALID, "block must be 64/128/256/512 and items 1/2/4/8");
+    const uint32_t b = t->block, i = t->items;
+    const bool ok = (b == 256 || b == 512) && (i == 2 || i == 4);
+    if (!ok) return ctx->fail(BZ_E_INVALID, "unsupported tuning: block 256 or 512, items 2 or 4");
     ctx->tune[kernel] = *t;
-    if (ctx->tune[kernel].grid_per_cu == 0) ctx->tune[kernel].grid_per_cu = 8;
     return BZ_OK;
 }
 
@@ -917,14 +921,13 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     // graph-owned device state
     hipError_t e = hipSuccess;
     const uint32_t cap = std::max<uint32_t>(tiles, 1u);
-    e = hipMalloc((void**)&g->lb.status, (size_t)cap * 8);
-    if (e == hipSuccess) e = hipMemset(g->lb.status, 0, (size_t)cap * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&g->lb.ticket, 64);
-    if (e == hipSuccess) e = hipMemset(g->lb.ticket, 0, 64);
+    e = hipMalloc((void**)&g->cs.counts, (size_t)cap * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&g->cs.staging, (size_t)cap * t.block * t.items * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc((void**)&g->err, 16);
+    if (e == hipSuccess) e = hipMemset(g->err, 0, 16);
     if (e != hipSuccess) return ctx->hip_fail(e, "graph scratch");
-    g->lb.done = g->lb.ticket + 1;
-    g->lb.error = g->lb.ticket + 2;
-    g->lb.capacity_tiles = cap;
+    g->cs.capacity_tiles = cap;
+    g->cs.capacity_entries = (uint64_t)cap * t.block * t.items;
     if (n_jobs) {
         e = hipMalloc((void**)&g->jobs_dev, cj.size() * sizeof(CullJob));
         if (e == hipSuccess) e = hipMemcpy(g->jobs_dev, cj.data(), cj.size() * sizeof(CullJob), hipMemcpyHostToDevice);
@@ -940,7 +943,7 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (e != hipSuccess) return ctx->hip_fail(e, "capture stream");
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess && !pj.empty())
-        e = launch_pyramid_batch(g->pjobs_dev, (uint32_t)pj.size(), wg, 0, g->lb.ticket + 3, cs);
+        e = launch_pyramid_batch(g->pjobs_dev, (uint32_t)pj.size(), wg, 0, g->err + 1, cs);
     for (auto& m : multi) {
         if (e != hipSuccess) break;
         Pyramid* p = m.first;
@@ -951,7 +954,7 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
                                      level_dim(p->base_w, L), level_dim(p->base_h, L), cs);
     }
     if (e == hipSuccess && n_jobs)
-        e = launch_cull_batch(g->jobs_dev, n_jobs, tiles, g->lb, t.block, t.items, ctx->batch_grid(t, tiles), cs);
+        e = launch_cull_batch(g->jobs_dev, n_jobs, tiles, g->cs, t.block, t.items, cs);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     (void)hipStreamDestroy(cs);
@@ -989,7 +992,7 @@ namespace bz {
 Graph::~Graph()
 {
     if (exec) (void)hipGraphExecDestroy(exec);
-    void* ps[] = { lb.status, lb.ticket, jobs_dev, pjobs_dev };
+    void* ps[] = { cs.counts, cs.staging, err, jobs_dev, pjobs_dev };
     for (void* p : ps)
         if (p) (void)hipFree(p);
 }

@@ -3,40 +3,24 @@
 // Replaces InitialDrawCull.comp.glsl:12-58, LateDrawCull.comp.glsl:13-71, TransparentDrawCull.comp.glsl:13-65,
 // OnpcDrawCull.comp.glsl:11-52 and drawOccTemporal.hlsl:13-64. The per-object arithmetic is in cull_math.h.
 //
-// Structure of one tile (BLOCK threads x ITEMS objects, objects tile*BLOCK*ITEMS + k*BLOCK + tid):
-//   1. all ITEMS objects' streams are loaded first (coalesced SoA float4 / uint32 loads), then decided;
-//   2. per item a wave64 ballot gives each emitting lane its rank inside the wave;
-//   3. wave counts are scanned in LDS in (item, wave) order, which is ascending object order;
-//   4. the tile's base in the output comes from a single-pass decoupled look-back over earlier tiles
-//      (tiles are numbered by an atomic ticket in launch order, so every wait is on a running workgroup);
-//   5. commands and visibility are written.
-// The result order is ascending objectId, independent of scheduling (the reference uses atomicAdd order).
+// Two kernels per pass, both HBM-streaming, with no wait between workgroups:
+//   decide: one workgroup per tile of BLOCK x ITEMS objects (object tile*TILE + k*BLOCK + tid). Every object stream
+//           of the tile is loaded at once (coalesced SoA float4 / uint32 loads); sphere -> view space and frustum
+//           test for every object; the frustum survivors of each wave are compacted into LDS so the Hi-Z test
+//           (two square roots, five divisions, a 2x2 gather) runs on dense lanes, and likewise the emitting objects
+//           for LOD selection. Emitting objects are ranked with a wave64 ballot and an LDS scan in (item, wave)
+//           order, i.e. ascending object order, and written as (objectId, LOD) to the tile's staging slot; the
+//           tile's count goes to counts[tile]. The LATE pass also writes visibility here.
+//   emit:   each workgroup takes a run of tiles, adds the counts of the job's earlier tiles (a block reduction) to
+//           get its output base, and writes DrawIndexedIndirect commands in order, plus the draw count.
+// The output order is ascending objectId, independent of scheduling (the reference appends in atomicAdd order).
+// A single-pass decoupled look-back was measured slower here: cross-XCD status visibility costs ~1 us per round
+// trip and the first generation of workgroups walked back tens of round trips.
 #include "bz_internal.h"
 
 namespace bz {
 
 namespace {
-
-constexpr unsigned long long kFlagAgg = 1ull << 62;
-constexpr unsigned long long kFlagInc = 2ull << 62;
-constexpr unsigned long long kFlagMask = 3ull << 62;
-constexpr uint32_t kSpinLimit = 1u << 24;  // bounded wait: ~seconds, then report an error and move on
-
-__device__ __forceinline__ unsigned long long ld_status(const unsigned long long* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(unsigned long long* p, unsigned long long v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
 {
@@ -48,131 +32,293 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
     return v;
 }
 
-// Decoupled look-back, executed by wave 0. `first_tile` is the first tile of this job in `status`.
-// Returns the exclusive prefix (number of commands emitted by earlier tiles of the job).
-__device__ __forceinline__ uint32_t lookback(unsigned long long* status, uint32_t tile, uint32_t first_tile,
-                                             uint32_t agg, uint32_t lane, uint32_t* err)
-{
-    if (tile == first_tile) {
-        if (lane == 0) st_status(&status[tile], kFlagInc | agg);
-        return 0;
-    }
-    if (lane == 0) st_status(&status[tile], kFlagAgg | agg);
-    uint32_t excl = 0;
-    int64_t look = (int64_t)tile - 1;
-    while (true) {
-        const int64_t idx = look - (int64_t)lane;
-        unsigned long long s = kFlagInc;  // before the job's first tile: inclusive 0
-        if (idx >= (int64_t)first_tile) {
-            uint32_t spins = 0;
-            s = ld_status(&status[idx]);
-            while ((s & kFlagMask) == 0) {
-                if (++spins > kSpinLimit) {
-                    atomicExch(err, 1u);
-                    s = kFlagInc;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                s = ld_status(&status[idx]);
-            }
-        }
-        const unsigned long long inc = __ballot((s & kFlagMask) == kFlagInc);
-        const uint32_t val = (uint32_t)s;
-        if (inc) {
-            const uint32_t first = (uint32_t)__ffsll((long long)inc) - 1u;  // nearest tile with an inclusive value
-            excl += wave_sum(lane <= first ? val : 0u);
-            break;
-        }
-        excl += wave_sum(val);
-        look -= 64;
-    }
-    if (lane == 0) st_status(&status[tile], kFlagInc | (excl + agg));
-    return excl;
-}
+// Small tables staged in LDS per workgroup when they fit: surface spheres and LOD ranges (512 surfaces, 12 KiB),
+// LOD errors (2048 LODs, 8 KiB) and the pyramid's level offsets. Larger tables are gathered from global memory.
+constexpr uint32_t kLdsSurfaces = 512;
+constexpr uint32_t kLdsLods = 2048;
 
 template <int BLOCK, int ITEMS>
 struct TileSmem {
+    float4 surf[kLdsSurfaces];
+    uint2 surf_lod[kLdsSurfaces];
+    float lod_err[kLdsLods];
+    float4 cand[BLOCK];               // per-wave compaction area: one chunk of 64 entries per wave
+    uint2 cand2[BLOCK];
+    uint32_t lvl[kMaxMips];
     uint32_t woff[ITEMS][BLOCK / 64];
-    uint32_t prefix;
-    uint32_t total;
-    uint32_t tile;
     uint32_t job;
-    uint32_t last;
 };
 
-// One tile of one job. Called by all threads of the workgroup.
-template <int BLOCK, int ITEMS>
-__device__ __forceinline__ void cull_tile(const CullJob& J, uint32_t tile_in_job, unsigned long long* status,
-                                          uint32_t first_tile, uint32_t* err, TileSmem<BLOCK, ITEMS>& sm)
+// Pass families compiled separately so the visibility stream is loaded unconditionally where it is used.
+enum : int { MODE_EARLY = 0, MODE_LATE = 1, MODE_OTHER = 2 };  // OTHER: PLAIN, TEMPORAL, ONPC (runtime-uniform)
+
+// Orders a wave's LDS writes before its later LDS reads (the compaction areas are private to a wave): a compiler
+// barrier, then a wait on LDS operations only (lgkmcnt(0); vmcnt and expcnt left at their maxima), so global loads in
+// flight, such as the next tile's prefetch, are not waited on.
+__device__ __forceinline__ void wave_lds_sync()
 {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Per-wave compaction of a lane predicate over ITEMS items: slot[k] = this lane's position among the wave's set
+// entries in (item, lane) order; returns the wave's total.
+template <int ITEMS>
+__device__ __forceinline__ uint32_t wave_compact(const bool (&pred)[ITEMS], uint32_t (&slot)[ITEMS], uint32_t lane)
+{
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const unsigned long long m = __ballot(pred[k]);
+        slot[k] = total + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        total += (uint32_t)__popcll(m);
+    }
+    return total;
+}
+
+// Object streams of one tile held in registers. Indices are clamped into the job so every load is issued without
+// waiting on another; out-of-range lanes are discarded afterwards.
+template <int ITEMS>
+struct Streams {
+    uint32_t gi[ITEMS], vis[ITEMS], surf[ITEMS];
+    bool valid[ITEMS], active[ITEMS];
+    float4 ps[ITEMS], q[ITEMS];
+};
+
+// Tile indices and the visibility stream (EARLY, LATE). Jobs always hold at least one object (empty ranges are
+// answered on the host), so clamped indices are always valid.
+template <int BLOCK, int ITEMS, int MODE>
+__device__ __forceinline__ void issue_vis(const CullJob& J, uint32_t tile_in_job, Streams<ITEMS>& s)
+{
+    const uint32_t base = tile_in_job * (uint32_t)(BLOCK * ITEMS);
+    const uint32_t nmax = J.n ? J.n - 1u : 0u;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = base + (uint32_t)k * BLOCK + threadIdx.x;
+        s.valid[k] = i < J.n;
+        s.gi[k] = J.begin + (s.valid[k] ? i : nmax);
+        s.vis[k] = 0u;
+    }
+    if (MODE == MODE_EARLY || MODE == MODE_LATE) {  // J.n >= 1: clamped indices are in range
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) s.vis[k] = J.vis_in[s.gi[k]];
+    }
+}
+
+// The surface-id and transform streams: for every object, or in EARLY only for last-frame-visible ones (which waits
+// for the visibility stream first).
+template <int ITEMS, int MODE, bool XF_ID>
+__device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& s)
+{
+    const SceneDev& S = J.scene;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        s.active[k] = s.valid[k] && (MODE != MODE_EARLY || s.vis[k] != 0u);
+        s.surf[k] = 0u;
+        s.ps[k] = s.q[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if (MODE != MODE_EARLY || s.active[k]) {
+            const uint32_t x = XF_ID ? s.gi[k] : S.obj_xf[s.gi[k]];
+            s.surf[k] = S.obj_surf[s.gi[k]];
+            s.ps[k] = S.xf_ps[x];
+            s.q[k] = S.xf_q[x];
+        }
+    }
+}
+
+template <int BLOCK, int ITEMS, int MODE, bool XF_ID>
+__device__ __forceinline__ void issue_tile(const CullJob& J, uint32_t tile_in_job, Streams<ITEMS>& s)
+{
+    issue_vis<BLOCK, ITEMS, MODE>(J, tile_in_job, s);
+    issue_objects<ITEMS, MODE, XF_ID>(J, s);
+}
+
+// Small tables into LDS (surface spheres and LOD ranges, LOD errors) plus the pyramid's level offsets. All loads of a
+// thread are issued before its first LDS write. The caller places the barrier.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void stage_tables(const CullJob& J, TileSmem<BLOCK, ITEMS>& sm, bool lds_tab)
+{
+    const uint32_t tid = threadIdx.x;
+    const SceneDev& S = J.scene;
+    if (lds_tab) {
+        constexpr int SPT = (kLdsSurfaces + BLOCK - 1) / BLOCK, LPT = (kLdsLods + BLOCK - 1) / BLOCK;
+        float4 t_sph[SPT];
+        uint2 t_lod[SPT];
+        float t_err[LPT];
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const uint32_t t = tid + (uint32_t)j * BLOCK;
+            const uint32_t tt = t < S.n_surfaces ? t : 0u;
+            t_sph[j] = S.surf_sphere[tt];
+            t_lod[j] = S.surf_lod[tt];
+        }
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            const uint32_t t = tid + (uint32_t)j * BLOCK;
+            t_err[j] = S.lod_error[t < S.n_lods ? t : 0u];
+        }
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const uint32_t t = tid + (uint32_t)j * BLOCK;
+            if (t < S.n_surfaces) {
+                sm.surf[t] = t_sph[j];
+                sm.surf_lod[t] = t_lod[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            const uint32_t t = tid + (uint32_t)j * BLOCK;
+            if (t < S.n_lods) sm.lod_err[t] = t_err[j];
+        }
+    }
+    if (tid < (uint32_t)kMaxMips) {  // level offsets from the extent (no load): sum of the smaller levels' areas
+        uint32_t off = 0;
+        for (uint32_t L = 0; L < tid; ++L) off += level_dim(J.pyr.base_w, L) * level_dim(J.pyr.base_h, L);
+        sm.lvl[tid] = off;
+    }
+}
+
+struct NoPrefetch {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// Decides one tile whose streams are in `s` (the decide kernel's body). Called by all threads of the workgroup.
+// `prefetch` is invoked once, after the tile's last global gather (the Hi-Z texels), so loads it issues stay in
+// flight through the rest of the tile (vmcnt retires in issue order: nothing later waits on them).
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, class Prefetch>
+__device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, const Streams<ITEMS>& s,
+                                             uint32_t* counts, uint2* staging, TileSmem<BLOCK, ITEMS>& sm,
+                                             Prefetch&& prefetch)
+{
+    constexpr bool lds_tab = LDS_TAB;
     constexpr int NW = BLOCK / 64;
+    constexpr uint32_t TILE = BLOCK * ITEMS;
+    constexpr bool late = MODE == MODE_LATE;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t pass = J.pass;
-    const bool uses_vis = (pass == PASS_EARLY || pass == PASS_LATE);
     const SceneDev& S = J.scene;
-    const uint32_t base = tile_in_job * (uint32_t)(BLOCK * ITEMS);
 
-    uint32_t vis_prev[ITEMS], surf[ITEMS];
-    float4 ps[ITEMS], q[ITEMS], sph[ITEMS];
-    bool active[ITEMS];
-
-    // ---- 1. loads
+    float4 sph[ITEMS];
+    if (lds_tab) {
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = base + (uint32_t)k * BLOCK + tid;
-        const bool valid = i < J.n;
-        const uint32_t gi = J.begin + i;
-        vis_prev[k] = (valid && uses_vis) ? J.vis_in[gi] : 0u;
-        active[k] = valid && (pass != PASS_EARLY || vis_prev[k] != 0u);
-        surf[k] = 0;
-        ps[k] = q[k] = sph[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (active[k]) {
-            const uint32_t x = S.obj_xf ? S.obj_xf[gi] : gi;
-            surf[k] = S.obj_surf[gi];
-            ps[k] = S.xf_ps[x];
-            q[k] = S.xf_q[x];
-        }
+        for (int k = 0; k < ITEMS; ++k) sph[k] = sm.surf[s.surf[k]];
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) sph[k] = S.surf_sphere[s.surf[k]];
     }
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k)
-        if (active[k]) sph[k] = S.surf_sphere[surf[k]];
 
-    // ---- 2. decisions
-    bool emit[ITEMS], vis_new[ITEMS];
-    uint2 draw[ITEMS];
-    uint32_t rank[ITEMS];
+    // ---- sphere -> view space + frustum for every object
+    float cx[ITEMS], cy[ITEMS], cz[ITEMS], rad[ITEMS];
+    bool vis_new[ITEMS], emit[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        emit[k] = false;
-        vis_new[k] = false;
-        draw[k] = make_uint2(0u, 0u);
-        if (active[k]) {
-            float cx, cy, cz, r;
-            sphere_to_view(J.view, sph[k].x, sph[k].y, sph[k].z, sph[k].w, ps[k].x, ps[k].y, ps[k].z, ps[k].w, q[k].x,
-                           q[k].y, q[k].z, q[k].w, cx, cy, cz, r);
-            bool visible = frustum_visible(J.view, cx, cy, cz, r);
-            if (J.hiz && visible) {
-                float a0, a1, a2, a3;
-                if (project_sphere(cx, cy, cz, r, J.view.znear, J.view.p00, J.view.p11, a0, a1, a2, a3))
-                    visible = occlusion_visible(J.view, J.pyr, a0, a1, a2, a3, cz, r);
+        sphere_to_view(J.view, sph[k].x, sph[k].y, sph[k].z, sph[k].w, s.ps[k].x, s.ps[k].y, s.ps[k].z, s.ps[k].w,
+                       s.q[k].x, s.q[k].y, s.q[k].z, s.q[k].w, cx[k], cy[k], cz[k], rad[k]);
+        vis_new[k] = s.active[k] && frustum_visible(J.view, cx[k], cy[k], cz[k], rad[k]);
+    }
+    float4* cand = sm.cand + wave * 64u;
+    uint2* cand2 = sm.cand2 + wave * 64u;
+
+    // ---- Hi-Z on the wave's frustum survivors, compacted onto dense lanes in chunks of 64. The texel gathers are
+    // issued by every lane (idle lanes read a clamped, in-bounds texel) and the prefetch is issued right after the
+    // first chunk's gathers: vmcnt retires in issue order, so waiting for the gathers does not wait for the prefetch.
+    if constexpr (HIZ) {
+        uint32_t slot[ITEMS];
+        const uint32_t n = wave_compact<ITEMS>(vis_new, slot, lane);
+        bool res[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) res[k] = vis_new[k];
+        auto chunk = [&](uint32_t c0, auto&& between) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k)
+                if (vis_new[k] && slot[k] - c0 < 64u) cand[slot[k] - c0] = make_float4(cx[k], cy[k], cz[k], rad[k]);
+            wave_lds_sync();
+            const bool act = c0 + lane < n;
+            const float4 c = cand[lane];
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+            const bool test =
+                act && project_sphere(c.x, c.y, c.z, c.w, J.view.znear, J.view.p00, J.view.p11, a0, a1, a2, a3);
+            const float width = (a2 - a0) * J.view.pyrW;
+            const float height = (a3 - a1) * J.view.pyrH;
+            const uint32_t L = hiz_level(glsl_max(width, height), J.pyr.mips);
+            const uint32_t W = level_dim(J.pyr.base_w, L), H = level_dim(J.pyr.base_h, L);
+            const Footprint f = footprint((a0 + a2) * 0.5f, (a1 + a3) * 0.5f, W, H);
+            const float* lvl = J.pyr.texels + sm.lvl[L];
+            const float d00 = lvl[(size_t)f.y0 * W + f.x0], d01 = lvl[(size_t)f.y0 * W + f.x1];
+            const float d10 = lvl[(size_t)f.y1 * W + f.x0], d11 = lvl[(size_t)f.y1 * W + f.x1];
+            __builtin_amdgcn_sched_barrier(0);  // keep the gathers ahead of the prefetch in issue order
+            between();
+            __builtin_amdgcn_sched_barrier(0);
+            if (act) {
+                const float d = glsl_min(glsl_min(d00, d01), glsl_min(d10, d11));
+                // a sphere crossing the near plane is not tested and stays visible
+                cand2[lane].x = (test ? (J.view.znear / (c.z - c.w) > d) : true) ? 1u : 0u;
             }
-            vis_new[k] = visible;
-            emit[k] = (pass == PASS_LATE) ? (visible && vis_prev[k] == 0u) : visible;
-            if (emit[k]) {
-                const uint2 lo = S.surf_lod[surf[k]];
-                const float len = sqrtf((cx * cx + cy * cy) + cz * cz);
-                const float distance = glsl_max(len - r, 0.0f);
-                const float threshold = (distance * J.view.lodTarget) / ps[k].w;
-                uint32_t idx = 0;
+            wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k)
+                if (vis_new[k] && slot[k] - c0 < 64u) res[k] = cand2[slot[k] - c0].x != 0u;
+            wave_lds_sync();
+        };
+        chunk(0u, prefetch);  // always runs (all lanes idle when n == 0): the prefetch is issued unconditionally
+        for (uint32_t c0 = 64u; c0 < n; c0 += 64u) chunk(c0, NoPrefetch{});
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) vis_new[k] = res[k];
+    } else {
+        prefetch();
+    }
+
+    // ---- LOD selection on the wave's emitting objects, compacted onto dense lanes in chunks of 64
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) emit[k] = late ? (vis_new[k] && s.vis[k] == 0u) : vis_new[k];
+    uint32_t eslot[ITEMS], lod_abs[ITEMS];
+    const uint32_t ne = wave_compact<ITEMS>(emit, eslot, lane);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) lod_abs[k] = 0u;
+    for (uint32_t c0 = 0; c0 < ne; c0 += 64u) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if (emit[k] && eslot[k] - c0 < 64u) {
+                cand[eslot[k] - c0] = make_float4(cx[k], cy[k], cz[k], rad[k]);
+                cand2[eslot[k] - c0] = make_uint2(__float_as_uint(s.ps[k].w), s.surf[k]);
+            }
+        wave_lds_sync();
+        if (c0 + lane < ne) {
+            const float4 c = cand[lane];
+            const uint2 e = cand2[lane];
+            const float scale = __uint_as_float(e.x);
+            const uint2 lo = lds_tab ? sm.surf_lod[e.y] : S.surf_lod[e.y];
+            const float len = sqrtf((c.x * c.x + c.y * c.y) + c.z * c.z);
+            const float distance = glsl_max(len - c.w, 0.0f);
+            const float threshold = (distance * J.view.lodTarget) / scale;
+            uint32_t idx = 0;
+            if (lds_tab) {
+                for (uint32_t l = 1; l < lo.y; ++l)
+                    if (sm.lod_err[lo.x + l] < threshold) idx = l;
+            } else {
                 for (uint32_t l = 1; l < lo.y; ++l)
                     if (S.lod_error[lo.x + l] < threshold) idx = l;
-                const uint32_t abs_idx = (pass == PASS_ONPC) ? idx : idx + lo.x;
-                draw[k] = S.lod_draw[abs_idx];
             }
+            cand2[lane].x = (pass == PASS_ONPC) ? idx : idx + lo.x;
         }
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if (emit[k] && eslot[k] - c0 < 64u) lod_abs[k] = cand2[eslot[k] - c0].x;
+        wave_lds_sync();
     }
 
-    // ---- 3. wave ballots + LDS scan in (item, wave) order
+    // ---- visibility; ranks in (item, wave) order -> staging
+    if (late) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if (s.valid[k]) J.vis_out[s.gi[k]] = vis_new[k] ? 1u : 0u;
+    }
+    uint32_t rank[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const unsigned long long m = __ballot(emit[k]);
@@ -184,146 +330,302 @@ __device__ __forceinline__ void cull_tile(const CullJob& J, uint32_t tile_in_job
         constexpr uint32_t E = ITEMS * NW;  // <= 64
         const uint32_t c = lane < E ? (&sm.woff[0][0])[lane] : 0u;
         const uint32_t incl = wave_incl_scan(c, lane);
-        const uint32_t agg = __shfl(incl, 63, 64);
         if (lane < E) (&sm.woff[0][0])[lane] = incl - c;
-        const uint32_t prefix = lookback(status, J.tile_start + tile_in_job, first_tile, agg, lane, err);
-        if (lane == 0) {
-            sm.prefix = prefix;
-            sm.total = prefix + agg;
-        }
+        if (lane == 63) counts[gtile] = incl;
     }
     __syncthreads();
-
-    // ---- 4. outputs
-    const uint32_t prefix = sm.prefix;
+    uint2* stage = staging + (size_t)gtile * TILE;
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = base + (uint32_t)k * BLOCK + tid;
-        if (emit[k]) {
-            const uint32_t pos = prefix + sm.woff[k][wave] + rank[k];
-            if (pos < J.capacity) {
-                uint2* c = reinterpret_cast<uint2*>(J.cmds + (size_t)pos * 6u);
-                c[0] = make_uint2(J.begin + i, draw[k].x);  // objectId, indexCount
-                c[1] = make_uint2(1u, draw[k].y);           // instanceCount, firstIndex
-                c[2] = make_uint2(0u, 0u);                  // vertexOffset, firstInstance
-            }
-        }
-        if (pass == PASS_LATE && i < J.n) J.vis_out[J.begin + i] = vis_new[k] ? 1u : 0u;
-    }
-    if (tid == 0 && tile_in_job + 1 == J.n_tiles) *J.count = sm.total;
+    for (int k = 0; k < ITEMS; ++k)
+        if (emit[k]) stage[sm.woff[k][wave] + rank[k]] = make_uint2(s.gi[k], lod_abs[k]);
+    __syncthreads();  // woff is rewritten by the next tile
 }
 
-// Last workgroup out clears the look-back state for the next launch. No memory fence is needed: every status
-// access is an agent-scope atomic, and a workgroup's look-back reads have returned before it counts itself done.
-// (An agent-scope release fence would write back the whole L2 of the XCD once per workgroup.)
-template <int BLOCK>
-__device__ __forceinline__ void finish_and_reset(const LookbackScratch& s, uint32_t n_status, uint32_t n_wg,
-                                                 uint32_t* sm_last)
+// One tile, loads and decision (non-persistent form, used by the batched kernel).
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
+__device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_job, uint32_t gtile, uint32_t* counts,
+                                            uint2* staging, TileSmem<BLOCK, ITEMS>& sm)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t d = atomicAdd(s.done, 1u);
-        *sm_last = (d + 1u == n_wg) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (*sm_last) {
-        for (uint32_t t = threadIdx.x; t < n_status; t += BLOCK) st_status(&s.status[t], 0ull);
-        if (threadIdx.x == 0) {
-            atomicExch(s.ticket, 0u);
-            atomicExch(s.done, 0u);
-        }
-    }
+    constexpr bool lds_tab = LDS_TAB;
+    Streams<ITEMS> s;
+    issue_vis<BLOCK, ITEMS, MODE>(J, tile_in_job, s);
+    if (MODE != MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID>(J, s);
+    stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
+    if (MODE == MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID>(J, s);
+    __syncthreads();  // staged tables visible
+    process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, gtile, s, counts, staging, sm, NoPrefetch{});
 }
 
-template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void cull_kernel(CullJob J, LookbackScratch s)
+// Persistent decide for one view: workgroup b takes tiles b, b + G, b + 2G, ... (no workgroup waits on another, so
+// any residency is safe). Tables are staged once; the next tile's streams are issued while the current one is
+// decided (two register sets, alternating, so no register copy waits on a load in flight).
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
+__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging)
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
-    if (threadIdx.x == 0) sm.tile = atomicAdd(s.ticket, 1u);
+    constexpr bool lds_tab = LDS_TAB;
+    const uint32_t G = gridDim.x;
+    uint32_t t = blockIdx.x;
+    Streams<ITEMS> A, B;
+    issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, t, A);
+    stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
     __syncthreads();
-    const uint32_t tile = sm.tile;
-    cull_tile<BLOCK, ITEMS>(J, tile, s.status, 0u, s.error, sm);
-    finish_and_reset<BLOCK>(s, J.n_tiles, gridDim.x, &sm.last);
+    while (true) {
+        {
+            const uint32_t tn = t + G;
+            // past the last tile the current one is issued again (L2 hits, discarded): no branch around the loads
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, A, counts, staging, sm, [&]() {
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, tn < J.n_tiles ? tn : t, B);
+            });
+            t = tn;
+            if (t >= J.n_tiles) break;
+        }
+        {
+            const uint32_t tn = t + G;
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, B, counts, staging, sm, [&]() {
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, tn < J.n_tiles ? tn : t, A);
+            });
+            t = tn;
+            if (t >= J.n_tiles) break;
+        }
+    }
 }
 
-// Persistent batched kernel: workgroups take (job, tile) items from a global ticket until all are done.
+__device__ __forceinline__ uint32_t find_job(const CullJob* jobs, uint32_t n_jobs, uint32_t t)
+{
+    uint32_t lo = 0, hi = n_jobs - 1;  // last job with tile_start <= t
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_start <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Batched decide: one launch over the (job, tile) work list of many views; workgroup b takes global tile b.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void cull_batch_kernel(const CullJob* __restrict__ jobs, uint32_t n_jobs,
-                                                           uint32_t total_tiles, LookbackScratch s)
+__global__ __launch_bounds__(BLOCK) void decide_batch_kernel(const CullJob* __restrict__ jobs, uint32_t n_jobs,
+                                                             uint32_t* counts, uint2* staging)
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
     __shared__ CullJob sjob;
-    uint32_t cached_job = 0xffffffffu;
-    while (true) {
-        if (threadIdx.x == 0) {
-            const uint32_t t = atomicAdd(s.ticket, 1u);
-            sm.tile = t;
-            if (t < total_tiles) {
-                // binary search: last job with tile_start <= t
-                uint32_t lo = 0, hi = n_jobs - 1;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1) >> 1;
-                    if (jobs[mid].tile_start <= t) lo = mid;
-                    else hi = mid - 1;
-                }
-                sm.job = lo;
+    const uint32_t t = blockIdx.x;
+    if (threadIdx.x == 0) sm.job = find_job(jobs, n_jobs, t);
+    __syncthreads();
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[sm.job]);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&sjob);
+    for (uint32_t w = threadIdx.x; w < sizeof(CullJob) / 4; w += BLOCK) dst[w] = src[w];
+    __syncthreads();
+    const uint32_t tt = t - sjob.tile_start;
+    const bool xf_id = sjob.scene.obj_xf == nullptr;
+    const bool lds = sjob.scene.n_surfaces <= kLdsSurfaces && sjob.scene.n_lods <= kLdsLods;
+    if (sjob.pass == PASS_LATE) {
+        if (!lds) decide_tile<BLOCK, ITEMS, MODE_LATE, true, false, false>(sjob, tt, t, counts, staging, sm);
+        else if (xf_id) decide_tile<BLOCK, ITEMS, MODE_LATE, true, true, true>(sjob, tt, t, counts, staging, sm);
+        else decide_tile<BLOCK, ITEMS, MODE_LATE, true, false, true>(sjob, tt, t, counts, staging, sm);
+    } else if (sjob.hiz) {
+        if (!lds) decide_tile<BLOCK, ITEMS, MODE_OTHER, true, false, false>(sjob, tt, t, counts, staging, sm);
+        else if (xf_id) decide_tile<BLOCK, ITEMS, MODE_OTHER, true, true, true>(sjob, tt, t, counts, staging, sm);
+        else decide_tile<BLOCK, ITEMS, MODE_OTHER, true, false, true>(sjob, tt, t, counts, staging, sm);
+    } else {
+        if (!lds) decide_tile<BLOCK, ITEMS, MODE_OTHER, false, false, false>(sjob, tt, t, counts, staging, sm);
+        else if (xf_id) decide_tile<BLOCK, ITEMS, MODE_OTHER, false, true, true>(sjob, tt, t, counts, staging, sm);
+        else decide_tile<BLOCK, ITEMS, MODE_OTHER, false, false, true>(sjob, tt, t, counts, staging, sm);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ emit
+constexpr uint32_t kEmitBlock = 256;
+constexpr uint32_t kEmitTiles = kEmitBlock / 64;  // one tile per wave
+constexpr uint32_t kEmitUnroll = 4;               // entries per lane in flight
+
+struct EmitSmem {
+    uint32_t red[kEmitBlock / 64];
+    uint32_t cnt[kEmitTiles + 1];     // exclusive prefix of the workgroup's tile counts
+    uint32_t job;
+};
+
+// Writes the commands of tiles [t0, t1) of one job (t1 - t0 <= kEmitTiles). One round of loads gets the counts of
+// the job's earlier tiles (reduced to the output base) and of the workgroup's own tiles; then wave w writes tile
+// t0 + w's entries in order, kEmitUnroll per lane with all their loads in flight.
+__device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32_t t1, uint32_t tile_size,
+                                           const uint32_t* counts, const uint2* staging, EmitSmem& sm)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t nt = t1 - t0;
+    // the job's earlier tiles: every load of a thread issued before the first add (16 per pass)
+    uint32_t part = 0;
+    const uint32_t own = tid < nt ? counts[t0 + tid] : 0u;
+    for (uint32_t u0 = J.tile_start; u0 < t0; u0 += 16u * kEmitBlock) {
+        uint32_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t u = u0 + (uint32_t)j * kEmitBlock + tid;
+            v[j] = u < t0 ? counts[u] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) part += v[j];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
+    if (lane == 0) sm.red[wave] = part;
+    if (wave == 0) {
+        uint32_t v = own;
+#pragma unroll
+        for (int off = 1; off < (int)kEmitTiles; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off) v += t;
+        }
+        if (lane < kEmitTiles) sm.cnt[lane + 1] = v;
+        if (lane == 0) sm.cnt[0] = 0;
+    }
+    __syncthreads();
+    uint32_t prefix = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kEmitBlock / 64u; ++w) prefix += sm.red[w];
+    if (tid == 0 && t1 == J.tile_start + J.n_tiles) *J.count = prefix + sm.cnt[nt];
+    if (wave >= nt) return;
+    const uint32_t c = sm.cnt[wave + 1] - sm.cnt[wave];
+    const uint32_t base = prefix + sm.cnt[wave];
+    const uint2* stage = staging + (size_t)(t0 + wave) * tile_size;
+    const uint2* lod_draw = J.scene.lod_draw;
+    for (uint32_t e0 = 0; e0 < c; e0 += 64u * kEmitUnroll) {
+        uint2 ent[kEmitUnroll], d[kEmitUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kEmitUnroll; ++u) {
+            const uint32_t e = e0 + u * 64u + lane;
+            ent[u] = stage[e < c ? e : 0u];  // objectId, absolute LOD (entry 0 exists whenever c > 0)
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kEmitUnroll; ++u) d[u] = lod_draw[ent[u].y];
+#pragma unroll
+        for (uint32_t u = 0; u < kEmitUnroll; ++u) {
+            const uint32_t e = e0 + u * 64u + lane;
+            const uint32_t pos = base + e;
+            if (e < c && pos < J.capacity) {
+                uint2* cmd = reinterpret_cast<uint2*>(J.cmds + (size_t)pos * 6u);
+                cmd[0] = make_uint2(ent[u].x, d[u].x);  // objectId, indexCount
+                cmd[1] = make_uint2(1u, d[u].y);         // instanceCount, firstIndex
+                cmd[2] = make_uint2(0u, 0u);             // vertexOffset, firstInstance
             }
         }
-        __syncthreads();
-        const uint32_t t = sm.tile;
-        if (t >= total_tiles) break;
-        const uint32_t j = sm.job;
-        if (j != cached_job) {
-            // stage the job descriptor in LDS (dword copy)
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[j]);
-            uint32_t* dst = reinterpret_cast<uint32_t*>(&sjob);
-            for (uint32_t w = threadIdx.x; w < sizeof(CullJob) / 4; w += BLOCK) dst[w] = src[w];
-            cached_job = j;
-            __syncthreads();
-        }
-        cull_tile<BLOCK, ITEMS>(sjob, t - sjob.tile_start, s.status, sjob.tile_start, s.error, sm);
-        __syncthreads();
     }
-    finish_and_reset<BLOCK>(s, total_tiles, gridDim.x, &sm.last);
+}
+
+__global__ __launch_bounds__(kEmitBlock) void emit_kernel(CullJob J, uint32_t tile_size, const uint32_t* counts,
+                                                          const uint2* staging)
+{
+    __shared__ EmitSmem sm;
+    const uint32_t t0 = blockIdx.x * kEmitTiles;
+    const uint32_t t1 = t0 + kEmitTiles < J.n_tiles ? t0 + kEmitTiles : J.n_tiles;
+    emit_tiles(J, t0, t1, tile_size, counts, staging, sm);
+}
+
+// Batched emit: workgroup b covers global tiles [b*kEmitTiles, ...), split at job boundaries.
+__global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* __restrict__ jobs, uint32_t n_jobs,
+                                                                uint32_t total_tiles, uint32_t tile_size,
+                                                                const uint32_t* counts, const uint2* staging)
+{
+    __shared__ EmitSmem sm;
+    uint32_t t = blockIdx.x * kEmitTiles;
+    const uint32_t t_end = t + kEmitTiles < total_tiles ? t + kEmitTiles : total_tiles;
+    while (t < t_end) {
+        __syncthreads();
+        if (threadIdx.x == 0) sm.job = find_job(jobs, n_jobs, t);
+        __syncthreads();
+        const CullJob& J = jobs[sm.job];
+        const uint32_t job_end = J.tile_start + J.n_tiles;
+        const uint32_t t1 = job_end < t_end ? job_end : t_end;
+        emit_tiles(J, t, t1, tile_size, counts, staging, sm);
+        t = t1;
+    }
+}
+
+// Persistent grid: `per_cu` workgroups per CU, or (per_cu == 0) as many as are resident at once per CU.
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
+void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st)
+{
+    static int resident = 0;
+    if (resident == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB>,
+                                                         BLOCK, 0) != hipSuccess ||
+            r < 1)
+            r = 1;
+        resident = r;
+    }
+    uint64_t grid = (uint64_t)n_cu * (per_cu ? per_cu : (uint32_t)resident);
+    if (grid > job.n_tiles) grid = job.n_tiles;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB>), dim3((uint32_t)grid), dim3(BLOCK), 0,
+                       st, job, counts, staging);
+}
+
+template <int BLOCK, int ITEMS, int MODE, bool HIZ>
+void launch_mode(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st)
+{
+    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
+    if (!lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, false, false>(job, n_cu, per_cu, counts, staging, st);
+    else if (job.scene.obj_xf == nullptr)
+        launch_decide<BLOCK, ITEMS, MODE, HIZ, true, true>(job, n_cu, per_cu, counts, staging, st);
+    else launch_decide<BLOCK, ITEMS, MODE, HIZ, false, true>(job, n_cu, per_cu, counts, staging, st);
 }
 
 template <int BLOCK, int ITEMS>
-hipError_t launch_cull_t(const CullJob& job, const LookbackScratch& s, hipStream_t st)
+hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n_cu, uint32_t per_cu, hipStream_t st)
 {
-    hipLaunchKernelGGL((cull_kernel<BLOCK, ITEMS>), dim3(job.n_tiles), dim3(BLOCK), 0, st, job, s);
+    const bool h = job.hiz != 0;
+    if (job.pass == PASS_EARLY) {
+        if (h) launch_mode<BLOCK, ITEMS, MODE_EARLY, true>(job, n_cu, per_cu, s.counts, s.staging, st);
+        else launch_mode<BLOCK, ITEMS, MODE_EARLY, false>(job, n_cu, per_cu, s.counts, s.staging, st);
+    } else if (job.pass == PASS_LATE) {
+        launch_mode<BLOCK, ITEMS, MODE_LATE, true>(job, n_cu, per_cu, s.counts, s.staging, st);
+    } else {
+        if (h) launch_mode<BLOCK, ITEMS, MODE_OTHER, true>(job, n_cu, per_cu, s.counts, s.staging, st);
+        else launch_mode<BLOCK, ITEMS, MODE_OTHER, false>(job, n_cu, per_cu, s.counts, s.staging, st);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
+                       (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts, (const uint2*)s.staging);
     return hipGetLastError();
 }
 
 template <int BLOCK, int ITEMS>
-hipError_t launch_cull_batch_t(const CullJob* jobs, uint32_t n_jobs, uint32_t total_tiles, const LookbackScratch& s,
-                               uint32_t grid, hipStream_t st)
+hipError_t launch_cull_batch_t(const CullJob* jobs, uint32_t n_jobs, uint32_t total_tiles, const CompactScratch& s,
+                               hipStream_t st)
 {
-    hipLaunchKernelGGL((cull_batch_kernel<BLOCK, ITEMS>), dim3(grid), dim3(BLOCK), 0, st, jobs, n_jobs, total_tiles, s);
+    hipLaunchKernelGGL((decide_batch_kernel<BLOCK, ITEMS>), dim3(total_tiles), dim3(BLOCK), 0, st, jobs, n_jobs,
+                       s.counts, s.staging);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(emit_batch_kernel, dim3((total_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st,
+                       jobs, n_jobs, total_tiles, (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts,
+                       (const uint2*)s.staging);
     return hipGetLastError();
 }
 
 }  // namespace
 
 #define BZ_TUNING_TABLE(X) \
-    X(64, 1) X(64, 2) X(64, 4) X(64, 8) \
-    X(128, 1) X(128, 2) X(128, 4) X(128, 8) \
-    X(256, 1) X(256, 2) X(256, 4) X(256, 8) \
-    X(512, 1) X(512, 2) X(512, 4) X(512, 8)
+    X(256, 2) X(256, 4) X(512, 2) X(512, 4)
 
-hipError_t launch_cull(const CullJob& job, const LookbackScratch& s, uint32_t block, uint32_t items, hipStream_t st)
+hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
+                       uint32_t per_cu, hipStream_t st)
 {
 #define X(B, I) \
-    if (block == B && items == I) return launch_cull_t<B, I>(job, s, st);
+    if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const LookbackScratch& s,
-                             uint32_t block, uint32_t items, uint32_t grid, hipStream_t st)
+hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const CompactScratch& s,
+                             uint32_t block, uint32_t items, hipStream_t st)
 {
+    if (n_jobs == 0 || total_tiles == 0) return hipSuccess;
 #define X(B, I) \
-    if (block == B && items == I) return launch_cull_batch_t<B, I>(jobs_dev, n_jobs, total_tiles, s, grid, st);
+    if (block == B && items == I) return launch_cull_batch_t<B, I>(jobs_dev, n_jobs, total_tiles, s, st);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;

@@ -159,8 +159,9 @@ __device__ __forceinline__ float footprint_min(const float* __restrict__ level, 
     return glsl_min(glsl_min(d00, d01), glsl_min(d10, d11));
 }
 
-__device__ __forceinline__ bool occlusion_visible(const ViewConsts& v, const PyramidDesc& p, float a0, float a1,
-                                                  float a2, float a3, float cz, float r)
+// level_offset: the pyramid's per-level texel offsets (kept in LDS by the caller).
+__device__ __forceinline__ bool occlusion_visible(const ViewConsts& v, const PyramidDesc& p, const uint32_t* level_offset,
+                                                  float a0, float a1, float a2, float a3, float cz, float r)
 {
     const float width = (a2 - a0) * v.pyrW;
     const float height = (a3 - a1) * v.pyrH;
@@ -169,7 +170,7 @@ __device__ __forceinline__ bool occlusion_visible(const ViewConsts& v, const Pyr
     const float w = (a1 + a3) * 0.5f;
     const uint32_t W = level_dim(p.base_w, L), H = level_dim(p.base_h, L);
     const Footprint f = footprint(u, w, W, H);
-    const float depth = footprint_min(p.texels + p.level_offset[L], W, f);
+    const float depth = footprint_min(p.texels + level_offset[L], W, f);
     const float depthSphere = v.znear / (cz - r);
     return depthSphere > depth;
 }

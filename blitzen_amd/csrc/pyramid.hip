@@ -5,8 +5,9 @@
 // footprint of its centre in the level above (level 0 samples the depth buffer), the rule in cull_math.h.
 //
 // Single pass for power-of-two bases: each workgroup owns a tile of level 0 (up to 32 x 32 texels), writes it,
-// and reduces it in LDS down to 1 x 1 (levels 1..5, exact 2:1 with edge clamp). The last workgroup to finish
-// (atomic counter, self-resetting) reduces the remaining levels from the 1-texel-per-tile image. Non-power-of-two
+// and reduces it in LDS down to 1 x 1 (levels 1..5, exact 2:1 with edge clamp). The last workgroup
+// (the highest-numbered one, waiting on flagged words) reduces the remaining levels from the
+// 1-texel-per-tile image. Non-power-of-two
 // bases fall back to one launch per level with the general footprint rule.
 #include "bz_internal.h"
 #include "synth_rng.h"
@@ -18,8 +19,8 @@ namespace {
 constexpr uint32_t kPyrBlock = 256;
 constexpr uint32_t kTile = 32;
 constexpr uint32_t kTailMax = 4096;  // texels of the per-tile image the last workgroup reduces in LDS
-constexpr unsigned long long kTailFlag = 1ull << 32;
-constexpr uint32_t kTailSpinLimit = 1u << 24;
+constexpr unsigned long long kTailFlag = 1ull << 63;
+constexpr uint32_t kTailSpinLimit = 1u << 20;
 
 __device__ __forceinline__ float min4(float a, float b, float c, float d) { return glsl_min(glsl_min(a, b), glsl_min(c, d)); }
 
@@ -70,21 +71,28 @@ __device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, f
         ch = nh;
     }
     if (J.tile_levels >= J.mips) return;
-    // ---- hand the tile's 1x1 value to the last workgroup. The value travels in a 64-bit agent-scope atomic word
-    // with a presence flag, so no release fence (an L2 write-back per workgroup) is needed: the last workgroup
-    // waits on each word's flag.
-    if (threadIdx.x == 0) {
-        const unsigned long long word = kTailFlag | (unsigned long long)__float_as_uint(src[0]);
-        __hip_atomic_store(&J.tail[tile], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t done = atomicAdd(J.counter, 1u);
-        *s_last = (done + 1u == J.tiles_x * J.tiles_y) ? 1u : 0u;
+    // ---- hand each tile's 1x1 value to the highest-numbered workgroup, which reduces the tail levels. The value
+    // travels in a 64-bit agent-scope atomic word with a presence flag; the receiving workgroup clears each word
+    // after reading it, so every build starts with all words zero. Neither a release fence (an L2 write-back per
+    // workgroup) nor a shared completion counter (a serialised atomic per workgroup) is needed.
+    const uint32_t n_tiles = J.tiles_x * J.tiles_y;
+    if (tile + 1u < n_tiles) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&J.tail[tile], kTailFlag | __float_as_uint(src[0]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return;
     }
-    __syncthreads();
-    if (!*s_last) return;
+    (void)s_last;
     const uint32_t L0 = J.tile_levels - 1u;
     cw = level_dim(J.base_w, L0);
     ch = level_dim(J.base_h, L0);
+    const float own = src[0];
+    __syncthreads();
     for (uint32_t t = threadIdx.x; t < cw * ch; t += kPyrBlock) {
+        if (t == tile) {
+            bufA[t] = own;
+            continue;
+        }
         unsigned long long w = __hip_atomic_load(&J.tail[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint32_t spins = 0;
         while (!(w & kTailFlag)) {
@@ -111,7 +119,6 @@ __device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, f
         cw = nw;
         ch = nh;
     }
-    if (threadIdx.x == 0) atomicExch(J.counter, 0u);
 }
 
 __global__ __launch_bounds__(kPyrBlock) void pyramid_kernel(PyrBuildJob J, uint32_t* err)

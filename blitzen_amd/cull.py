@@ -194,7 +194,7 @@ class Context:
         cap = cmds.nbytes // 24 if capacity is None else capacity
         return cmds.download(INDIRECT_DRAW, min(n, cap), stream), n
 
-    def set_tuning(self, kernel: int, block: int, items: int, grid_per_cu: int = 8):
+    def set_tuning(self, kernel: int, block: int, items: int, grid_per_cu: int = 0):
         t = A.Tuning(block=block, items=items, grid_per_cu=grid_per_cu)
         self.check(lib().bz_set_tuning(self.handle, kernel, C.byref(t)), "bz_set_tuning")
 

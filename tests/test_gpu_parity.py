@@ -210,7 +210,7 @@ def test_shared_transforms_and_out_of_place_visibility(ctx):
     ctx.scene_destroy(h)
 
 
-@pytest.mark.parametrize("tuning", [(64, 1), (128, 2), (256, 8), (512, 4), (512, 8)])
+@pytest.mark.parametrize("tuning", [(256, 2), (256, 4), (512, 2), (512, 4)])
 def test_tunings_identical(ctx, tuning):
     n = 70000
     sc = H.c1_scene(n)
