@@ -1,0 +1,201 @@
+"""Pins the oracle's whole passes (CPU only): an independent float64 numpy restatement of the plain pass with
+margins at the decision boundaries, the relations the five pass modes must satisfy with one another, and the
+emission contract (ascending objectId, exact count past capacity, ranges, threads).
+
+Pass semantics: InitialDrawCull.comp.glsl:12-58 (EARLY), LateDrawCull.comp.glsl:13-71 (LATE),
+TransparentDrawCull.comp.glsl:13-65 (PLAIN), drawOccTemporal.hlsl:13-64 (TEMPORAL), OnpcDrawCull.comp.glsl:11-52.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from blitzen_amd import _capi as A
+from oracle import pyoracle as orc
+from tests import helpers as H
+
+BW, BH, MIPS = 1024, 512, 11
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return H.as_tuple(H.c0_scene())
+
+
+@pytest.fixture(scope="module")
+def depth_pyr():
+    v = H.view()
+    d = H.c1_depth(seed=H.SEED_C0, occluders=256, p11=v.proj5)
+    tex = orc.pyramid_build(d, BW, BH, MIPS)
+    return orc.pyramid(tex, BW, BH, MIPS)
+
+
+def const_pyr(value):
+    n = orc.load().orc_pyramid_texels(BW, BH, MIPS, None)
+    return orc.pyramid(np.full(n, value, np.float32), BW, BH, MIPS)
+
+
+def quat_mats(q):
+    x, y, z, w = (q[:, k].astype(np.float64) for k in range(4))
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def plain_float64(scene, vb):
+    """Frustum + LOD of every object in float64. Returns (visible, ambiguous, lod_rel, lod_ambiguous)."""
+    objects, transforms, surfaces, lods = scene
+    t = transforms[objects["transformId"]]
+    s = surfaces[objects["surfaceId"]]
+    f = orc.view_floats(vb).astype(np.float64)
+    M = f[:16].reshape(4, 4).T  # column-major storage: M[row, col] = view[4*col + row]
+    world = np.einsum("nij,nj->ni", quat_mats(t["orientation"]), s["center"].astype(np.float64))
+    world = world * t["scale"][:, None] + t["pos"]
+    c = world @ M[:3, :3].T + M[:3, 3]
+    r = s["radius"].astype(np.float64) * t["scale"]
+    fR, fL, fT, fB, zn, zf = f[35], f[36], f[37], f[38], f[41], f[42]
+    d = np.stack([c[:, 2] * fL - np.abs(c[:, 0]) * fR + r, c[:, 2] * fB - np.abs(c[:, 1]) * fT + r,
+                  c[:, 2] + r - zn, zf - (c[:, 2] - r)], -1)
+    visible = (d > 0).all(-1)
+    ambiguous = (np.abs(d) < 1e-3 * (1 + np.abs(c).max(-1, keepdims=True))).any(-1)
+    dist = np.maximum(np.linalg.norm(c, axis=1) - r, 0.0)
+    thr = dist * f[45] / t["scale"]
+    lod_rel = np.zeros(len(objects), np.int64)
+    lod_amb = np.zeros(len(objects), bool)
+    errs = lods["error"].astype(np.float64)
+    for k in range(1, int(s["lodCount"].max())):
+        has = k < s["lodCount"]
+        e = errs[np.minimum(s["lodOffset"] + k, len(lods) - 1)]
+        lod_rel = np.where(has & (e < thr), k, lod_rel)
+        lod_amb |= has & (np.abs(e - thr) < 1e-5 * np.maximum(thr, 1e-9))
+    return visible, ambiguous, lod_rel, lod_amb
+
+
+def run(scene, vb, pass_, pyr=None, flags=0, vis_in=None, threads=4, **kw):
+    n = len(scene[0])
+    vis_out = np.zeros(n, np.uint32) if pass_ == A.PASS_LATE else None
+    cmds, count, lod = orc.cull(scene, vb, pass_, pyr, flags, vis_in=vis_in, vis_out=vis_out, threads=threads,
+                                want_lod=True, **kw)
+    return cmds, count, lod, vis_out
+
+
+def test_plain_pass_matches_float64(scene):
+    for pose in [dict(), dict(pos=(10.0, -20.0, 5.0), yaw=1.3, pitch=0.4), dict(fov=50.0, w=512, h=512, yaw=-2.5)]:
+        v = H.view(pyr=(BW, BH), **pose)
+        v.lodTarget = 1e-4
+        vb = H.view_bytes(v)
+        cmds, count, lod, _ = run(scene, vb, A.PASS_PLAIN)
+        visible, amb, lod_rel, lod_amb = plain_float64(scene, vb)
+        assert count == len(cmds)
+        ids = cmds[:, 0]
+        got = np.zeros(len(visible), bool)
+        got[ids] = True
+        assert (got == visible)[~amb].all(), np.nonzero((got != visible) & ~amb)[0][:10]
+        assert amb.mean() < 0.01 and 0.02 < visible.mean() < 0.9
+        surf = scene[2][scene[0]["surfaceId"][ids]]
+        ok = ~lod_amb[ids]
+        np.testing.assert_array_equal((lod - surf["lodOffset"])[ok], lod_rel[ids][ok])
+        assert len(np.unique(lod_rel[ids])) >= 3  # several LODs exercised
+
+
+def test_command_fields(scene):
+    vb = H.view_bytes(H.view(pyr=(BW, BH)))
+    cmds, count, lod, _ = run(scene, vb, A.PASS_PLAIN)
+    lods = scene[3]
+    assert (np.diff(cmds[:, 0].astype(np.int64)) > 0).all()
+    np.testing.assert_array_equal(cmds[:, 1], lods["indexCount"][lod])
+    np.testing.assert_array_equal(cmds[:, 2], 1)
+    np.testing.assert_array_equal(cmds[:, 3], lods["firstIndex"][lod])
+    np.testing.assert_array_equal(cmds[:, 4:], 0)
+    surf = scene[2][scene[0]["surfaceId"][cmds[:, 0]]]
+    assert ((lod >= surf["lodOffset"]) & (lod < surf["lodOffset"] + surf["lodCount"])).all()
+
+
+def ids_of(cmds, n):
+    m = np.zeros(n, bool)
+    m[cmds[:, 0]] = True
+    return m
+
+
+def test_pass_relations(scene, depth_pyr):
+    n = len(scene[0])
+    vb = H.view_bytes(H.view(pyr=(BW, BH)))
+    vis_prev = H.bz.synth_visibility(H.SEED_C0, n, 0.5)
+    plain, _, plain_lod, _ = run(scene, vb, A.PASS_PLAIN)
+    temporal, _, temporal_lod, _ = run(scene, vb, A.PASS_TEMPORAL, depth_pyr)
+    early, _, _, _ = run(scene, vb, A.PASS_EARLY, vis_in=vis_prev)
+    early_hiz, _, _, _ = run(scene, vb, A.PASS_EARLY, depth_pyr, flags=A.CULL_EARLY_HIZ, vis_in=vis_prev)
+    late, _, _, vis_late = run(scene, vb, A.PASS_LATE, depth_pyr, vis_in=vis_prev)
+    onpc, _, onpc_lod, _ = run(scene, vb, A.PASS_ONPC)
+    P, T = ids_of(plain, n), ids_of(temporal, n)
+    prev = vis_prev.astype(bool)
+    assert (T & ~P).sum() == 0 and 0 < T.sum() < P.sum()  # Hi-Z only removes, and removes some
+    np.testing.assert_array_equal(ids_of(early, n), P & prev)
+    np.testing.assert_array_equal(ids_of(early_hiz, n), T & prev)
+    np.testing.assert_array_equal(ids_of(late, n), T & ~prev)
+    np.testing.assert_array_equal(vis_late.astype(bool), T)
+    np.testing.assert_array_equal(onpc[:, 0], plain[:, 0])
+    surf = scene[2][scene[0]["surfaceId"][plain[:, 0]]]
+    np.testing.assert_array_equal(onpc_lod, plain_lod - surf["lodOffset"])  # relative index, no offset
+    # LOD choice does not depend on the pass
+    common = np.intersect1d(plain[:, 0], temporal[:, 0], return_indices=True)
+    np.testing.assert_array_equal(plain_lod[common[1]], temporal_lod[common[2]])
+
+
+def test_hiz_with_constant_pyramids(scene):
+    n = len(scene[0])
+    vb = H.view_bytes(H.view(pyr=(BW, BH)))
+    plain = ids_of(run(scene, vb, A.PASS_PLAIN)[0], n)
+    far = ids_of(run(scene, vb, A.PASS_TEMPORAL, const_pyr(0.0))[0], n)      # reversed-z: 0 is infinitely far
+    near = ids_of(run(scene, vb, A.PASS_TEMPORAL, const_pyr(1.0))[0], n)     # 1 is the near plane: all behind
+    np.testing.assert_array_equal(far, plain)
+    # only spheres crossing the near plane (not projectable) survive a wall at the near plane
+    objects, transforms, surfaces, _ = scene
+    f = orc.view_floats(vb)
+    t = transforms[objects["transformId"]]
+    s = surfaces[objects["surfaceId"]]
+    z = (np.einsum("nij,nj->ni", quat_mats(t["orientation"]), s["center"]) * t["scale"][:, None] + t["pos"])[:, 2]
+    r = s["radius"] * t["scale"]
+    crossing = z < r + f[41]
+    assert (near & ~(plain & crossing)).sum() == 0
+
+
+def test_late_visibility_in_place(scene, depth_pyr):
+    n = len(scene[0])
+    vb = H.view_bytes(H.view(pyr=(BW, BH)))
+    vis = H.bz.synth_visibility(H.SEED_C0 + 1, n, 0.3)
+    out_sep = np.zeros(n, np.uint32)
+    a, ca = orc.cull(scene, vb, A.PASS_LATE, depth_pyr, vis_in=vis.copy(), vis_out=out_sep, threads=3)
+    vv = vis.copy()
+    b, cb = orc.cull(scene, vb, A.PASS_LATE, depth_pyr, vis_in=vv, vis_out=vv, threads=3)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(out_sep, vv)
+
+
+@pytest.mark.parametrize("threads", [0, 1, 2, 7, 16])
+def test_threads_do_not_change_output(scene, depth_pyr, threads):
+    n = len(scene[0])
+    vb = H.view_bytes(H.view(pyr=(BW, BH)))
+    vis = H.bz.synth_visibility(H.SEED_C0, n, 0.5)
+    ref = run(scene, vb, A.PASS_LATE, depth_pyr, vis_in=vis, threads=0)
+    got = run(scene, vb, A.PASS_LATE, depth_pyr, vis_in=vis, threads=threads)
+    for x, y in zip(ref, got):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_capacity_and_ranges(scene):
+    n = len(scene[0])
+    vb = H.view_bytes(H.view(pyr=(BW, BH)))
+    full, total, _, _ = run(scene, vb, A.PASS_PLAIN)
+    part, cnt, _, _ = run(scene, vb, A.PASS_PLAIN, capacity=100)
+    assert cnt == total and len(part) == 100
+    np.testing.assert_array_equal(part, full[:100])
+    zero, cnt0, _, _ = run(scene, vb, A.PASS_PLAIN, capacity=0)
+    assert cnt0 == total and len(zero) == 0
+    for b, c in [(0, 1), (1, 1), (5, 1000), (n - 7, 7), (4096, 4096), (123, n - 123)]:
+        sub, k, _, _ = run(scene, vb, A.PASS_PLAIN, begin=b, count=c)
+        sel = full[(full[:, 0] >= b) & (full[:, 0] < b + c)]
+        assert k == len(sel)
+        np.testing.assert_array_equal(sub, sel)
+    with pytest.raises(ValueError):
+        run(scene, vb, A.PASS_PLAIN, begin=n - 1, count=2)
