@@ -103,8 +103,9 @@ struct Ctx {
     bz_status make_job(uint32_t pass, uint32_t flags, bz_scene sh, bz_pyramid ph, const bz_CameraViewData& view,
                        uint32_t begin, uint32_t count, const uint32_t* vis_in, uint32_t* vis_out, void* cmds,
                        uint32_t* count_dev, uint32_t capacity, uint32_t block, uint32_t items, CullJob* job);
+    // Jobs with objects go to `out` (tile_start set); the count buffers of jobs with none go to `empty`.
     bz_status make_batch(const bz_view_job* jobs, uint32_t n_jobs, const bz_tuning& t, std::vector<CullJob>& out,
-                         uint32_t* total_tiles);
+                         std::vector<uint32_t*>& empty, uint32_t* total_tiles);
     uint32_t batch_grid(const bz_tuning& t, uint32_t total_tiles) const;
     bz_status upload_jobs(StreamScratch* sc, const CullJob* jobs, uint32_t n, hipStream_t st);
     bz_status upload_pjobs(StreamScratch* sc, const PyrBuildJob* jobs, uint32_t n, hipStream_t st);

@@ -798,24 +798,40 @@ bz_status Ctx::upload_pjobs(StreamScratch* sc, const PyrBuildJob* jobs, uint32_t
 }
 
 bz_status Ctx::make_batch(const bz_view_job* jobs, uint32_t n_jobs, const bz_tuning& t, std::vector<CullJob>& out,
-                          uint32_t* total_tiles)
+                          std::vector<uint32_t*>& empty, uint32_t* total_tiles)
 {
-    out.resize(n_jobs);
+    out.clear();
+    out.reserve(n_jobs);
+    empty.clear();
     uint64_t tiles = 0;
     for (uint32_t k = 0; k < n_jobs; ++k) {
         const bz_view_job& v = jobs[k];
         if (v.pass != PASS_LATE && v.pass != PASS_PLAIN && v.pass != PASS_TEMPORAL)
             return fail(BZ_E_INVALID, "batched jobs support the LATE, PLAIN and TEMPORAL passes");
+        CullJob j;
         bz_status s = make_job(v.pass, 0, v.scene, v.pyramid, v.view, v.object_begin, v.object_count,
                                v.visibility_in_dev, v.visibility_out_dev, v.cmds_dev, v.count_dev, v.capacity, t.block,
-                               t.items, &out[k]);
+                               t.items, &j);
         if (s) return s;
-        out[k].tile_start = (uint32_t)tiles;
-        tiles += out[k].n_tiles;
+        if (j.n == 0) {  // the kernels assume every job has objects; an empty one only gets its count zeroed
+            empty.push_back(j.count);
+            continue;
+        }
+        j.tile_start = (uint32_t)tiles;
+        tiles += j.n_tiles;
         if (tiles > 0x7fffffffull) return fail(BZ_E_INVALID, "batch too large");
+        out.push_back(j);
     }
     *total_tiles = (uint32_t)tiles;
     return BZ_OK;
+}
+
+static hipError_t zero_counts(const std::vector<uint32_t*>& empty, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    for (uint32_t* c : empty)
+        if (e == hipSuccess) e = launch_fill_u32(c, 0u, 1, st);
+    return e;
 }
 
 uint32_t Ctx::batch_grid(const bz_tuning& t, uint32_t total_tiles) const
@@ -833,21 +849,25 @@ bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, v
     if (!jobs) return ctx->fail(BZ_E_INVALID, "NULL jobs");
     const bz_tuning t = ctx->tune[1];
     std::vector<CullJob> cj;
+    std::vector<uint32_t*> empty;
     uint32_t tiles = 0;
-    bz_status s = ctx->make_batch(jobs, n_jobs, t, cj, &tiles);
+    bz_status s = ctx->make_batch(jobs, n_jobs, t, cj, empty, &tiles);
     if (s) return s;
     hipStream_t st = (hipStream_t)stream;
-    StreamScratch* sc = nullptr;
-    s = ctx->scratch_for(st, tiles, (uint64_t)tiles * t.block * t.items, &sc);
-    if (s) return s;
     {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
             return ctx->fail(BZ_E_STATE, "bz_cull_batch cannot be stream-captured; use bz_batch_graph_create");
     }
-    s = ctx->upload_jobs(sc, cj.data(), n_jobs, st);
+    hipError_t e = zero_counts(empty, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "count fill");
+    if (cj.empty()) return BZ_OK;
+    StreamScratch* sc = nullptr;
+    s = ctx->scratch_for(st, tiles, (uint64_t)tiles * t.block * t.items, &sc);
     if (s) return s;
-    hipError_t e = launch_cull_batch(sc->jobs_dev, n_jobs, tiles, sc->cs, t.block, t.items, st);
+    s = ctx->upload_jobs(sc, cj.data(), (uint32_t)cj.size(), st);
+    if (s) return s;
+    e = launch_cull_batch(sc->jobs_dev, (uint32_t)cj.size(), tiles, sc->cs, t.block, t.items, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "batched cull launch");
     return BZ_OK;
@@ -900,8 +920,9 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (!g) return BZ_E_OOM;
     const bz_tuning t = ctx->tune[1];
     std::vector<CullJob> cj;
+    std::vector<uint32_t*> empty;
     uint32_t tiles = 0;
-    bz_status s = n_jobs ? ctx->make_batch(jobs, n_jobs, t, cj, &tiles) : BZ_OK;
+    bz_status s = n_jobs ? ctx->make_batch(jobs, n_jobs, t, cj, empty, &tiles) : BZ_OK;
     if (s) return s;
     std::vector<PyrBuildJob> pj;
     std::vector<std::pair<Pyramid*, bz_pyramid_job>> multi;
@@ -928,7 +949,7 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (e != hipSuccess) return ctx->hip_fail(e, "graph scratch");
     g->cs.capacity_tiles = cap;
     g->cs.capacity_entries = (uint64_t)cap * t.block * t.items;
-    if (n_jobs) {
+    if (!cj.empty()) {
         e = hipMalloc((void**)&g->jobs_dev, cj.size() * sizeof(CullJob));
         if (e == hipSuccess) e = hipMemcpy(g->jobs_dev, cj.data(), cj.size() * sizeof(CullJob), hipMemcpyHostToDevice);
     }
@@ -953,8 +974,9 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
                                      level_dim(p->base_h, L - 1), p->texels + p->level_offset[L],
                                      level_dim(p->base_w, L), level_dim(p->base_h, L), cs);
     }
-    if (e == hipSuccess && n_jobs)
-        e = launch_cull_batch(g->jobs_dev, n_jobs, tiles, g->cs, t.block, t.items, cs);
+    if (e == hipSuccess) e = zero_counts(empty, cs);
+    if (e == hipSuccess && !cj.empty())
+        e = launch_cull_batch(g->jobs_dev, (uint32_t)cj.size(), tiles, g->cs, t.block, t.items, cs);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     (void)hipStreamDestroy(cs);

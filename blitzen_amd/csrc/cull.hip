@@ -419,19 +419,23 @@ __global__ __launch_bounds__(BLOCK) void decide_batch_kernel(const CullJob* __re
     const uint32_t tt = t - sjob.tile_start;
     const bool xf_id = sjob.scene.obj_xf == nullptr;
     const bool lds = sjob.scene.n_surfaces <= kLdsSurfaces && sjob.scene.n_lods <= kLdsLods;
-    if (sjob.pass == PASS_LATE) {
-        if (!lds) decide_tile<BLOCK, ITEMS, MODE_LATE, true, false, false>(sjob, tt, t, counts, staging, sm);
-        else if (xf_id) decide_tile<BLOCK, ITEMS, MODE_LATE, true, true, true>(sjob, tt, t, counts, staging, sm);
-        else decide_tile<BLOCK, ITEMS, MODE_LATE, true, false, true>(sjob, tt, t, counts, staging, sm);
-    } else if (sjob.hiz) {
-        if (!lds) decide_tile<BLOCK, ITEMS, MODE_OTHER, true, false, false>(sjob, tt, t, counts, staging, sm);
-        else if (xf_id) decide_tile<BLOCK, ITEMS, MODE_OTHER, true, true, true>(sjob, tt, t, counts, staging, sm);
-        else decide_tile<BLOCK, ITEMS, MODE_OTHER, true, false, true>(sjob, tt, t, counts, staging, sm);
-    } else {
-        if (!lds) decide_tile<BLOCK, ITEMS, MODE_OTHER, false, false, false>(sjob, tt, t, counts, staging, sm);
-        else if (xf_id) decide_tile<BLOCK, ITEMS, MODE_OTHER, false, true, true>(sjob, tt, t, counts, staging, sm);
-        else decide_tile<BLOCK, ITEMS, MODE_OTHER, false, false, true>(sjob, tt, t, counts, staging, sm);
+    // identity scenes have no transform-index stream (obj_xf == nullptr): XF_ID must follow it in every variant
+#define BZ_DECIDE_TILE(M, HZ)                                                                                     \
+    if (xf_id) {                                                                                                 \
+        if (lds) decide_tile<BLOCK, ITEMS, M, HZ, true, true>(sjob, tt, t, counts, staging, sm);                 \
+        else decide_tile<BLOCK, ITEMS, M, HZ, true, false>(sjob, tt, t, counts, staging, sm);                    \
+    } else {                                                                                                     \
+        if (lds) decide_tile<BLOCK, ITEMS, M, HZ, false, true>(sjob, tt, t, counts, staging, sm);                \
+        else decide_tile<BLOCK, ITEMS, M, HZ, false, false>(sjob, tt, t, counts, staging, sm);                   \
     }
+    if (sjob.pass == PASS_LATE) {
+        BZ_DECIDE_TILE(MODE_LATE, true)
+    } else if (sjob.hiz) {
+        BZ_DECIDE_TILE(MODE_OTHER, true)
+    } else {
+        BZ_DECIDE_TILE(MODE_OTHER, false)
+    }
+#undef BZ_DECIDE_TILE
 }
 
 // ------------------------------------------------------------------------------------------------ emit
@@ -564,11 +568,16 @@ void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t*
 template <int BLOCK, int ITEMS, int MODE, bool HIZ>
 void launch_mode(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st)
 {
+    // identity scenes have no transform-index stream (obj_xf == nullptr): XF_ID must follow it in every variant
     const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
-    if (!lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, false, false>(job, n_cu, per_cu, counts, staging, st);
-    else if (job.scene.obj_xf == nullptr)
-        launch_decide<BLOCK, ITEMS, MODE, HIZ, true, true>(job, n_cu, per_cu, counts, staging, st);
-    else launch_decide<BLOCK, ITEMS, MODE, HIZ, false, true>(job, n_cu, per_cu, counts, staging, st);
+    const bool xf_id = job.scene.obj_xf == nullptr;
+    if (xf_id) {
+        if (lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, true, true>(job, n_cu, per_cu, counts, staging, st);
+        else launch_decide<BLOCK, ITEMS, MODE, HIZ, true, false>(job, n_cu, per_cu, counts, staging, st);
+    } else {
+        if (lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, false, true>(job, n_cu, per_cu, counts, staging, st);
+        else launch_decide<BLOCK, ITEMS, MODE, HIZ, false, false>(job, n_cu, per_cu, counts, staging, st);
+    }
 }
 
 template <int BLOCK, int ITEMS>

@@ -207,6 +207,36 @@ class Context:
         arr = (A.ViewJob * len(jobs))(*jobs)
         self.check(lib().bz_cull_batch(self.handle, arr, len(jobs), stream), "bz_cull_batch")
 
+    @staticmethod
+    def view_job(scene: int, view: CameraViewData, pass_: int, cmds: DeviceBuffer, count: DeviceBuffer,
+                 capacity: int | None = None, pyramid: int = 0, begin: int = 0, n: int = 0, vis_in: int | None = None,
+                 vis_out: int | None = None) -> A.ViewJob:
+        if capacity is None:
+            capacity = cmds.nbytes // 24
+        return A.ViewJob(scene=scene, pyramid=pyramid, pass_=pass_, object_begin=begin, object_count=n,
+                         capacity=capacity, view=view, visibility_in_dev=vis_in, visibility_out_dev=vis_out,
+                         cmds_dev=cmds.ptr, count_dev=count.ptr)
+
+    def pyramid_build_batch(self, jobs: list[tuple[int, int, int, int]], stream=None):
+        """jobs: (pyramid, depth_dev, w, h) each."""
+        arr = (A.PyramidJob * len(jobs))(*[A.PyramidJob(pyramid=p, depth_dev=d, w=w, h=h) for p, d, w, h in jobs])
+        self.check(lib().bz_pyramid_build_batch(self.handle, arr, len(jobs), stream), "bz_pyramid_build_batch")
+
+    def batch_graph_create(self, pyr_jobs: list[tuple[int, int, int, int]], jobs: list[A.ViewJob]) -> int:
+        parr = (A.PyramidJob * max(len(pyr_jobs), 1))(*[A.PyramidJob(pyramid=p, depth_dev=d, w=w, h=h)
+                                                         for p, d, w, h in pyr_jobs])
+        jarr = (A.ViewJob * max(len(jobs), 1))(*jobs)
+        g = C.c_uint64()
+        self.check(lib().bz_batch_graph_create(self.handle, parr, len(pyr_jobs), jarr, len(jobs), C.byref(g)),
+                   "bz_batch_graph_create")
+        return g.value
+
+    def graph_launch(self, graph: int, stream=None):
+        self.check(lib().bz_graph_launch(self.handle, graph, stream), "bz_graph_launch")
+
+    def graph_destroy(self, graph: int):
+        self.check(lib().bz_graph_destroy(self.handle, graph), "bz_graph_destroy")
+
     def synth_depth_device(self, seed: int, w: int, h: int, occluders: int, p11: float, znear: float, depth_dev: int,
                            stream=None):
         self.check(lib().bz_synth_depth_device(self.handle, seed, w, h, occluders, p11, znear, depth_dev, stream),

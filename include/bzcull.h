@@ -277,19 +277,22 @@ typedef struct bz_view_job {
 } bz_view_job;
 
 typedef struct bz_tuning {
-    uint32_t block;           /* threads per workgroup: 64, 128, 256, 512 */
-    uint32_t items;           /* objects per thread:    1, 2, 4, 8          */
-    uint32_t grid_per_cu;     /* persistent workgroups per CU (batched kernel) */
+    uint32_t block;           /* threads per workgroup: 256 or 512 */
+    uint32_t items;           /* objects per thread:    2 or 4     */
+    uint32_t grid_per_cu;     /* workgroups per CU for the persistent decide kernel; 0 = as many as fit */
     uint32_t reserved;
 } bz_tuning;
 
-/* Uploads the job table and launches one persistent kernel for all jobs. The job table is staged in a
- * context-owned device buffer; a call may be captured into a hipGraph (see bz_batch_graph_*). */
+/* Culls every job in one launch pair (decide + emit over the jobs' concatenated tiles). Each job's output
+ * equals what bz_cull gives for it alone. Jobs with no objects only get their count zeroed. The job table
+ * is staged through a context-owned pinned buffer, so this call cannot be stream-captured: capture a fixed
+ * batch with bz_batch_graph_create instead. Jobs that write one visibility buffer in overlapping ranges race. */
 BZ_API bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, void* stream);
-/* Tuning used by bz_cull / bz_cull_batch (defaults are built in; bz_autotune replaces them). */
 /* Synchronizes `stream` and reports BZ_E_INTERNAL if a kernel launched on it gave up a bounded wait
  * since the last check (the flag is then cleared). */
 BZ_API bz_status bz_check(bz_ctx* ctx, void* stream);
+/* Tuning used by bz_cull (kernel 0) and bz_cull_batch (kernel 1); defaults are built in, bz_autotune
+ * replaces them. */
 BZ_API bz_status bz_get_tuning(bz_ctx* ctx, uint32_t kernel /* 0 = single view, 1 = batched */, bz_tuning* out);
 BZ_API bz_status bz_set_tuning(bz_ctx* ctx, uint32_t kernel, const bz_tuning* t);
 /* Times every candidate tuning on the given jobs (results are identical for every candidate) and keeps
