@@ -15,6 +15,7 @@ namespace bz {
 
 struct Scene {
     uint32_t n_objects = 0, n_transforms = 0, n_surfaces = 0, n_lods = 0, n_clusters = 0;
+    uint32_t max_lod_clusters = 0;   // largest LodData.clusterCount (set with the cluster table)
     bool xf_identity = false;
     uint32_t* obj_surf = nullptr;
     uint32_t* obj_xf = nullptr;
@@ -62,6 +63,7 @@ struct StreamScratch {
     PyrBuildJob* pjobs_host = nullptr;
     uint32_t pjobs_cap = 0;
     hipEvent_t upload_done = nullptr; // last job-table upload from the pinned buffers
+    ClusterScratch xs{};              // cluster pass
     ~StreamScratch();
 };
 
@@ -100,6 +102,7 @@ struct Ctx {
     Scene* scene(bz_scene h);
     Pyramid* pyramid(bz_pyramid h);
     bz_status scratch_for(hipStream_t st, uint32_t tiles, uint64_t entries, StreamScratch** out);
+    bz_status cluster_scratch_for(hipStream_t st, StreamScratch* sc, uint32_t tiles, uint64_t records);
     bz_status make_job(uint32_t pass, uint32_t flags, bz_scene sh, bz_pyramid ph, const bz_CameraViewData& view,
                        uint32_t begin, uint32_t count, const uint32_t* vis_in, uint32_t* vis_out, void* cmds,
                        uint32_t* count_dev, uint32_t capacity, uint32_t block, uint32_t items, CullJob* job);

@@ -83,22 +83,37 @@ hipError_t launch_synth_depth(uint64_t seed, uint32_t w, uint32_t h, uint32_t oc
                               float* depth, hipStream_t st);
 
 // Cluster two-level cull
+// Cluster pass of the two-level cluster cull (the object pass is a CullJob run by launch_cluster_objects).
 struct ClusterJob {
     SceneDev scene;
     ViewConsts view;
     PyramidDesc pyr;
     const float4* cl_sphere;    // [n_clusters] center, radius
-    const uint32_t* cl_cone;    // [n_clusters] int8 x, y, z, cutoff packed
+    const uint32_t* cl_cone;    // [n_clusters] int8 axis x, y, z and cutoff, packed low to high
     const uint2* cl_draw;       // [n_clusters] indexCount (= triangleCount*3), firstIndex (= dataOffset)
-    uint32_t* dispatch;         // ClusterDispatchData[dispatch_capacity] (3 dwords each)
+    uint32_t* dispatch;         // ClusterDispatchData[dispatch_capacity] as 3 dwords each
     uint32_t* dispatch_count;
-    uint32_t* cmds;
+    uint32_t* cmds;             // IndirectDrawData[capacity] as 6 dwords each
     uint32_t* count;
     uint32_t dispatch_capacity, capacity;
-    uint32_t tests, hiz;
-    uint32_t* work_offsets;     // scratch: per object-tile exclusive offsets into the record list
+    uint32_t tests, hiz;        // BZ_CLUSTER_* and whether a pyramid is bound
+    uint32_t n_tiles, tile_size;
 };
-hipError_t launch_cluster_cull(const ClusterJob& job, const CompactScratch& s, uint32_t* expand_total_dev,
+
+// Scratch of the cluster pass: per object tile the cluster total (written by the object pass), the first record
+// and the survivor count; survivors (objectId, clusterId) compacted inside each tile's record range.
+struct ClusterScratch {
+    uint32_t* ccounts;          // [capacity_tiles]
+    uint32_t* rbase;            // [capacity_tiles]
+    uint32_t* scount;           // [capacity_tiles]
+    uint2* survivors;           // [capacity_records]
+    uint32_t capacity_tiles;
+    uint64_t capacity_records;
+};
+
+hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, uint32_t block,
+                                  uint32_t items, hipStream_t st);
+hipError_t launch_cluster_pass(const ClusterJob& job, const CompactScratch& cs, const ClusterScratch& xs,
                                hipStream_t st);
 
 }  // namespace bz

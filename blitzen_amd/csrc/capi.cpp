@@ -80,7 +80,7 @@ Pyramid::~Pyramid()
 
 StreamScratch::~StreamScratch()
 {
-    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err };
+    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err, xs.ccounts, xs.rbase, xs.scount, xs.survivors };
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (jobs_host) (void)hipHostFree(jobs_host);
@@ -122,6 +122,31 @@ bz_status Ctx::scratch_for(hipStream_t st, uint32_t tiles, uint64_t entries, Str
         s.cs.capacity_entries = cap_e;
     }
     *out = &s;
+    return BZ_OK;
+}
+
+// Cluster-pass scratch: per-tile arrays for `tiles` object tiles and a survivor buffer of `records` entries.
+bz_status Ctx::cluster_scratch_for(hipStream_t st, StreamScratch* sc, uint32_t tiles, uint64_t records)
+{
+    ClusterScratch& x = sc->xs;
+    if (x.capacity_tiles >= tiles && x.capacity_records >= records && x.ccounts) return BZ_OK;
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cst) == hipSuccess && cst != hipStreamCaptureStatusNone)
+        return fail(BZ_E_STATE, "cluster scratch must grow, which is not allowed during stream capture");
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    const uint32_t cap_t = std::max<uint32_t>(std::max(tiles, x.capacity_tiles), 1u << 10);
+    const uint64_t cap_r = std::max<uint64_t>(std::max(records, x.capacity_records), 1ull << 16);
+    for (void* p : { (void*)x.ccounts, (void*)x.rbase, (void*)x.scount, (void*)x.survivors })
+        if (p) (void)hipFree(p);
+    x = ClusterScratch{};
+    e = hipMalloc((void**)&x.ccounts, (size_t)cap_t * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&x.rbase, (size_t)cap_t * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&x.scount, (size_t)cap_t * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&x.survivors, (size_t)cap_r * sizeof(uint2));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(cluster scratch)");
+    x.capacity_tiles = cap_t;
+    x.capacity_records = cap_r;
     return BZ_OK;
 }
 
@@ -397,9 +422,12 @@ bz_status bz_scene_set_clusters(bz_ctx* ctx, bz_scene scene, const bz_Cluster* c
     std::vector<uint2> lcl(s->n_lods);
     if (hipMemcpy(lcl.data(), s->lod_cluster, lcl.size() * sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
         return ctx->fail(BZ_E_HIP, "hipMemcpy(lod clusters)");
-    for (uint32_t l = 0; l < s->n_lods; ++l)
+    uint32_t max_cl = 0;
+    for (uint32_t l = 0; l < s->n_lods; ++l) {
         if (lcl[l].y && (uint64_t)lcl[l].x + lcl[l].y > n_clusters)
             return ctx->fail(BZ_E_INVALID, "LOD " + std::to_string(l) + " cluster range exceeds the cluster table");
+        max_cl = std::max(max_cl, lcl[l].y);
+    }
     std::vector<float4> sph(n_clusters);
     std::vector<uint32_t> cone(n_clusters);
     std::vector<uint2> draw(n_clusters);
@@ -416,6 +444,7 @@ bz_status bz_scene_set_clusters(bz_ctx* ctx, bz_scene scene, const bz_Cluster* c
     s->cl_cone = nullptr;
     s->cl_draw = nullptr;
     s->n_clusters = n_clusters;
+    s->max_lod_clusters = n_clusters ? max_cl : 0;
     if (!n_clusters) return BZ_OK;
     hipError_t e = hipMalloc((void**)&s->cl_sphere, sph.size() * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc((void**)&s->cl_cone, cone.size() * 4);
@@ -745,6 +774,62 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
     if (s) return s;
     hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "cull launch");
+    return BZ_OK;
+}
+
+// ---------------------------------------------------------------------------- cluster cull
+bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* stream)
+{
+    if (!ctx || !a || !a->view) return BZ_E_INVALID;
+    Scene* s = ctx->scene(a->scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (!s->cl_sphere) return ctx->fail(BZ_E_STATE, "the scene has no cluster table (bz_scene_set_clusters)");
+    if (a->cluster_tests & ~(BZ_CLUSTER_FRUSTUM | BZ_CLUSTER_HIZ | BZ_CLUSTER_CONE))
+        return ctx->fail(BZ_E_INVALID, "unknown cluster test bits");
+    if (!a->dispatch_count_dev || !a->count_dev) return ctx->fail(BZ_E_INVALID, "NULL count buffer");
+    if (a->dispatch_capacity && !a->dispatch_dev) return ctx->fail(BZ_E_INVALID, "NULL dispatch buffer");
+    if ((uintptr_t)a->dispatch_dev % 4u) return ctx->fail(BZ_E_INVALID, "dispatch_dev must be 4-byte aligned");
+    if ((uint64_t)s->n_objects * s->max_lod_clusters > 0xffffffffull)
+        return ctx->fail(BZ_E_INVALID, "the scene can expand to more than 2^32-1 cluster records");
+    const bz_tuning& t = ctx->tune[0];
+    if (t.block * t.items > 2048u) return ctx->fail(BZ_E_INVALID, "object tile larger than 2048");
+    CullJob oj;
+    bz_status st_ = ctx->make_job(a->pyramid ? PASS_TEMPORAL : PASS_PLAIN, 0, a->scene, a->pyramid, *a->view, 0, 0,
+                                  nullptr, nullptr, a->cmds_dev, a->count_dev, a->capacity, t.block, t.items, &oj);
+    if (st_) return st_;
+    hipStream_t st = (hipStream_t)stream;
+    if (oj.n == 0) {
+        hipError_t e = launch_fill_u32(a->dispatch_count_dev, 0u, 1, st);
+        if (e == hipSuccess) e = launch_fill_u32(a->count_dev, 0u, 1, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "count fill");
+        return BZ_OK;
+    }
+    StreamScratch* sc = nullptr;
+    st_ = ctx->scratch_for(st, oj.n_tiles, (uint64_t)oj.n_tiles * t.block * t.items, &sc);
+    if (st_) return st_;
+    st_ = ctx->cluster_scratch_for(st, sc, oj.n_tiles, a->dispatch_capacity);
+    if (st_) return st_;
+    ClusterJob cj;
+    std::memset(&cj, 0, sizeof cj);
+    cj.scene = oj.scene;
+    cj.view = oj.view;
+    cj.pyr = oj.pyr;
+    cj.cl_sphere = s->cl_sphere;
+    cj.cl_cone = s->cl_cone;
+    cj.cl_draw = s->cl_draw;
+    cj.dispatch = reinterpret_cast<uint32_t*>(a->dispatch_dev);
+    cj.dispatch_count = a->dispatch_count_dev;
+    cj.cmds = reinterpret_cast<uint32_t*>(a->cmds_dev);
+    cj.count = a->count_dev;
+    cj.dispatch_capacity = a->dispatch_capacity;
+    cj.capacity = a->capacity;
+    cj.tests = a->cluster_tests;
+    cj.hiz = oj.hiz;
+    cj.n_tiles = oj.n_tiles;
+    cj.tile_size = t.block * t.items;
+    hipError_t e = launch_cluster_objects(oj, sc->cs, sc->xs.ccounts, t.block, t.items, st);
+    if (e == hipSuccess) e = launch_cluster_pass(cj, sc->cs, sc->xs, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "cluster cull launch");
     return BZ_OK;
 }
 

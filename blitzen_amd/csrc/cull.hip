@@ -46,6 +46,7 @@ struct TileSmem {
     uint2 cand2[BLOCK];
     uint32_t lvl[kMaxMips];
     uint32_t woff[ITEMS][BLOCK / 64];
+    uint32_t csum[BLOCK / 64];        // cluster mode: per-wave sums of the emitting objects' cluster counts
     uint32_t job;
 };
 
@@ -189,10 +190,11 @@ struct NoPrefetch {
 // Decides one tile whose streams are in `s` (the decide kernel's body). Called by all threads of the workgroup.
 // `prefetch` is invoked once, after the tile's last global gather (the Hi-Z texels), so loads it issues stay in
 // flight through the rest of the tile (vmcnt retires in issue order: nothing later waits on them).
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, class Prefetch>
+// CL (cluster object pass): ccounts[gtile] also receives the sum of the emitting objects' LOD cluster counts.
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, bool CL = false, class Prefetch>
 __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, const Streams<ITEMS>& s,
-                                             uint32_t* counts, uint2* staging, TileSmem<BLOCK, ITEMS>& sm,
-                                             Prefetch&& prefetch)
+                                             uint32_t* counts, uint2* staging, uint32_t* ccounts,
+                                             TileSmem<BLOCK, ITEMS>& sm, Prefetch&& prefetch)
 {
     constexpr bool lds_tab = LDS_TAB;
     constexpr int NW = BLOCK / 64;
@@ -312,6 +314,16 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
         wave_lds_sync();
     }
 
+    if constexpr (CL) {  // the tile's cluster total, for the expansion's record offsets
+        uint32_t cc = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if (emit[k]) cc += S.lod_cluster[lod_abs[k]].y;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) cc += __shfl_xor(cc, off, 64);
+        if (lane == 0) sm.csum[wave] = cc;
+    }
+
     // ---- visibility; ranks in (item, wave) order -> staging
     if (late) {
 #pragma unroll
@@ -332,6 +344,14 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
         const uint32_t incl = wave_incl_scan(c, lane);
         if (lane < E) (&sm.woff[0][0])[lane] = incl - c;
         if (lane == 63) counts[gtile] = incl;
+        if constexpr (CL) {
+            if (lane == 0) {
+                uint32_t tot = 0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) tot += sm.csum[w];
+                ccounts[gtile] = tot;
+            }
+        }
     }
     __syncthreads();
     uint2* stage = staging + (size_t)gtile * TILE;
@@ -341,10 +361,10 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
     __syncthreads();  // woff is rewritten by the next tile
 }
 
-// One tile, loads and decision (non-persistent form, used by the batched kernel).
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
+// One tile, loads and decision (non-persistent form, used by the batched kernel and the cluster object pass).
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CL = false>
 __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_job, uint32_t gtile, uint32_t* counts,
-                                            uint2* staging, TileSmem<BLOCK, ITEMS>& sm)
+                                            uint2* staging, TileSmem<BLOCK, ITEMS>& sm, uint32_t* ccounts = nullptr)
 {
     constexpr bool lds_tab = LDS_TAB;
     Streams<ITEMS> s;
@@ -353,7 +373,7 @@ __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_j
     stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
     if (MODE == MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID>(J, s);
     __syncthreads();  // staged tables visible
-    process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, gtile, s, counts, staging, sm, NoPrefetch{});
+    process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, CL>(J, gtile, s, counts, staging, ccounts, sm, NoPrefetch{});
 }
 
 // Persistent decide for one view: workgroup b takes tiles b, b + G, b + 2G, ... (no workgroup waits on another, so
@@ -374,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
         {
             const uint32_t tn = t + G;
             // past the last tile the current one is issued again (L2 hits, discarded): no branch around the loads
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, A, counts, staging, sm, [&]() {
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, A, counts, staging, nullptr, sm, [&]() {
                 issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, tn < J.n_tiles ? tn : t, B);
             });
             t = tn;
@@ -382,7 +402,7 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
         }
         {
             const uint32_t tn = t + G;
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, B, counts, staging, sm, [&]() {
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, B, counts, staging, nullptr, sm, [&]() {
                 issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, tn < J.n_tiles ? tn : t, A);
             });
             t = tn;
@@ -436,6 +456,45 @@ __global__ __launch_bounds__(BLOCK) void decide_batch_kernel(const CullJob* __re
         BZ_DECIDE_TILE(MODE_OTHER, false)
     }
 #undef BZ_DECIDE_TILE
+}
+
+// Cluster object pass (PreClusterDrawCull.comp.glsl:13-47): the PLAIN (or, with a pyramid, TEMPORAL) decision of
+// every object, its LOD, and per tile the number of emitting objects (counts), their (objectId, LOD) list (staging)
+// and the sum of their LOD cluster counts (ccounts). One workgroup per tile.
+template <int BLOCK, int ITEMS, bool HIZ, bool XF_ID, bool LDS_TAB>
+__global__ __launch_bounds__(BLOCK) void decide_cluster_kernel(CullJob J, uint32_t* counts, uint2* staging,
+                                                               uint32_t* ccounts)
+{
+    __shared__ TileSmem<BLOCK, ITEMS> sm;
+    decide_tile<BLOCK, ITEMS, MODE_OTHER, HIZ, XF_ID, LDS_TAB, true>(J, blockIdx.x, blockIdx.x, counts, staging, sm,
+                                                                     ccounts);
+}
+
+template <int BLOCK, int ITEMS>
+hipError_t launch_cluster_objects_t(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, hipStream_t st)
+{
+    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
+    const bool xf_id = job.scene.obj_xf == nullptr;
+    const dim3 g(job.n_tiles), b(BLOCK);
+#define BZ_CL_LAUNCH(HZ)                                                                                           \
+    if (xf_id) {                                                                                                   \
+        if (lds) hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, true, true>), g, b, 0, st, job,      \
+                                    s.counts, s.staging, ccounts);                                                 \
+        else hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, true, false>), g, b, 0, st, job,         \
+                                s.counts, s.staging, ccounts);                                                     \
+    } else {                                                                                                       \
+        if (lds) hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, false, true>), g, b, 0, st, job,     \
+                                    s.counts, s.staging, ccounts);                                                 \
+        else hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, false, false>), g, b, 0, st, job,        \
+                                s.counts, s.staging, ccounts);                                                     \
+    }
+    if (job.hiz) {
+        BZ_CL_LAUNCH(true)
+    } else {
+        BZ_CL_LAUNCH(false)
+    }
+#undef BZ_CL_LAUNCH
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------------ emit
@@ -624,6 +683,16 @@ hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t blo
 {
 #define X(B, I) \
     if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st);
+    BZ_TUNING_TABLE(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, uint32_t block,
+                                  uint32_t items, hipStream_t st)
+{
+#define X(B, I) \
+    if (block == B && items == I) return launch_cluster_objects_t<B, I>(job, s, ccounts, st);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;

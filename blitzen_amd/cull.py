@@ -189,6 +189,19 @@ class Context:
                        visibility_in_dev=vis_in, visibility_out_dev=vis_out)
         self.check(lib().bz_cull(self.handle, C.byref(a), stream), "bz_cull")
 
+    def cull_clusters(self, scene: int, view: CameraViewData, tests: int, dispatch: DeviceBuffer,
+                      dispatch_count: DeviceBuffer, cmds: DeviceBuffer, count: DeviceBuffer, pyramid: int = 0,
+                      dispatch_capacity: int | None = None, capacity: int | None = None, stream=None):
+        if dispatch_capacity is None:
+            dispatch_capacity = dispatch.nbytes // 12
+        if capacity is None:
+            capacity = cmds.nbytes // 24
+        a = A.ClusterCullArgs(scene=scene, pyramid=pyramid, cluster_tests=tests, flags=0, view=C.pointer(view),
+                              dispatch_dev=dispatch.ptr, dispatch_count_dev=dispatch_count.ptr,
+                              dispatch_capacity=dispatch_capacity, cmds_dev=cmds.ptr, count_dev=count.ptr,
+                              capacity=capacity)
+        self.check(lib().bz_cull_clusters(self.handle, C.byref(a), stream), "bz_cull_clusters")
+
     def read_commands(self, cmds: DeviceBuffer, count: DeviceBuffer, capacity: int | None = None, stream=None):
         n = int(count.download(np.uint32, 1, stream)[0])
         cap = cmds.nbytes // 24 if capacity is None else capacity
