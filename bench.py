@@ -75,6 +75,77 @@ def cpu_baseline(scene, vb, depth, vis0, budget_s: float):
             "ms_per_frame": med * 1e3}
 
 
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))]
+
+
+def bench_batched(ctx, stream, steps: int, warmup: int):
+    """256 views of uneven size: one captured graph (pyramid builds + batched cull) vs per-view launches."""
+    import numpy as np
+
+    from blitzen_amd import workloads as W
+    from blitzen_amd.hip import Event
+
+    t0 = time.perf_counter()
+    plan = W.batched_plan()
+    bv = W.BatchedViews.build(ctx, plan, stream=stream)
+    log(f"bench: batched workload ({len(plan)} views, {bv.n_objects} object-view pairs) built in "
+        f"{time.perf_counter() - t0:.1f}s")
+    tuned = ctx.autotune(1, bv.jobs, stream=stream)
+    tuned0 = ctx.autotune(0, bv.jobs, stream=stream)
+    graph = ctx.batch_graph_create(bv.pyr_jobs, bv.jobs)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        ctx.sync(stream)
+        evs = [(Event(), Event()) for _ in range(steps)]
+        t_start = time.perf_counter()
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        ctx.sync(stream)
+        wall = (time.perf_counter() - t_start) / steps
+        ctx.check_kernels(stream)
+        return [a.elapsed_ms(b) for a, b in evs], wall
+
+    t_graph, wall_graph = timed(lambda: ctx.graph_launch(graph, stream))
+    c_graph = bv.counts(stream).copy()
+    t_plain, wall_plain = timed(lambda: bv.run_per_view(stream))
+    c_plain = bv.counts(stream)
+    if not np.array_equal(c_graph, c_plain):
+        raise RuntimeError("batched and per-view draw counts differ")
+    ctx.graph_destroy(graph)
+    n_views = len(plan)
+    cmds = int(c_graph.sum())
+    alg = W.algorithmic_bytes_batched(plan, cmds)
+    g_ms, p_ms = statistics.median(t_graph), statistics.median(t_plain)
+    out = {
+        "workload": f"{n_views} views, per-view objects log-uniform in [1024, 1048576] ({bv.n_objects} object-view "
+                    f"pairs), per-view depth 512x512 / 1024x1024 / 2048x1024 with 64-1024 occluders, per-view "
+                    f"pyramid built in the same launch sequence, late-pass semantics",
+        "views": n_views,
+        "object_view_pairs": bv.n_objects,
+        "views_per_s": round(n_views / (g_ms * 1e-3), 1),
+        "object_culls_per_s": round(bv.n_objects / (g_ms * 1e-3), 1),
+        "graph_ms_p50": round(g_ms, 4),
+        "graph_ms_p99": round(pct(t_graph, 99), 4),
+        "graph_wall_ms": round(wall_graph * 1e3, 4),
+        "achieved_gbs": round(alg / (g_ms * 1e-3) / 1e9, 1),
+        "algorithmic_bytes": alg,
+        "per_view_launches": {"views_per_s": round(n_views / (p_ms * 1e-3), 1), "ms_p50": round(p_ms, 4),
+                              "ms_p99": round(pct(t_plain, 99), 4), "wall_ms": round(wall_plain * 1e3, 4)},
+        "speedup_vs_per_view": round(p_ms / g_ms, 2),
+        "tuning": {"batched": [tuned.block, tuned.items], "single_view": [tuned0.block, tuned0.items,
+                                                                          tuned0.grid_per_cu]},
+        "commands": cmds,
+    }
+    bv.free()
+    return out
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_late_cull_c1.json"
     if p.exists():
@@ -93,6 +164,7 @@ def main():
     ap.add_argument("--copies", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -218,6 +290,9 @@ def main():
         "commands": {"early": n_early, "late": n_late},
         "roofline": roofline,
     }
+    if not args.no_batched:
+        log("bench: batched views ...")
+        result["batched"] = bench_batched(ctx, stream, max(5, min(args.steps, 20)), 2)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline((sc.objects, sc.transforms, sc.surfaces, sc.lods), bytes(memoryview(v)),

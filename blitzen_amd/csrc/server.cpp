@@ -21,6 +21,5 @@ bz_status bz_server_fetch(bz_ctx* ctx, uint64_t, bz_IndirectDrawData*, uint32_t,
 bz_status bz_server_cancel(bz_ctx* ctx, uint64_t) { return ctx ? BZ_E_STATE : BZ_E_INVALID; }
 bz_status bz_server_release(bz_ctx* ctx, uint64_t) { return ctx ? BZ_E_STATE : BZ_E_INVALID; }
 bz_status bz_server_get_stats(bz_ctx* ctx, bz_server_stats*) { return ctx ? BZ_E_STATE : BZ_E_INVALID; }
-bz_status bz_autotune(bz_ctx* ctx, uint32_t, const bz_view_job*, uint32_t, const char*, bz_tuning*, void*) { return ctx ? BZ_E_STATE : BZ_E_INVALID; }
 
 }

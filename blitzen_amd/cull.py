@@ -211,6 +211,13 @@ class Context:
         t = A.Tuning(block=block, items=items, grid_per_cu=grid_per_cu)
         self.check(lib().bz_set_tuning(self.handle, kernel, C.byref(t)), "bz_set_tuning")
 
+    def autotune(self, kernel: int, jobs: list[A.ViewJob], cache_path: str | None = None, stream=None) -> A.Tuning:
+        arr = (A.ViewJob * len(jobs))(*jobs)
+        t = A.Tuning()
+        self.check(lib().bz_autotune(self.handle, kernel, arr, len(jobs),
+                                     cache_path.encode() if cache_path else None, C.byref(t), stream), "bz_autotune")
+        return t
+
     def get_tuning(self, kernel: int) -> A.Tuning:
         t = A.Tuning()
         self.check(lib().bz_get_tuning(self.handle, kernel, C.byref(t)), "bz_get_tuning")

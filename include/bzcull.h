@@ -295,9 +295,11 @@ BZ_API bz_status bz_check(bz_ctx* ctx, void* stream);
  * replaces them. */
 BZ_API bz_status bz_get_tuning(bz_ctx* ctx, uint32_t kernel /* 0 = single view, 1 = batched */, bz_tuning* out);
 BZ_API bz_status bz_set_tuning(bz_ctx* ctx, uint32_t kernel, const bz_tuning* t);
-/* Times every candidate tuning on the given jobs (results are identical for every candidate) and keeps
- * the fastest. If cache_path is not NULL the result is read from / written to that JSON file, keyed by
- * device name, kernel and object-count bucket. */
+/* Times every candidate tuning (block 256/512 x items 2/4, and for kernel 0 a persistent grid of 0, 1, 2
+ * or 4 workgroups per CU) on the given jobs with HIP events and keeps the fastest; results are identical for
+ * every candidate. The jobs run repeatedly, so a LATE job should write its visibility out of place. If
+ * cache_path is not NULL the choice is read from / written to that JSON file, keyed by device name, kernel
+ * and floor(log2(total objects)); a cached entry is used without timing. */
 BZ_API bz_status bz_autotune(bz_ctx* ctx, uint32_t kernel, const bz_view_job* jobs, uint32_t n_jobs,
                       const char* cache_path, bz_tuning* best, void* stream);
 

@@ -257,3 +257,30 @@ def test_single_view_large_tables(ctx, scenes):
         j.free()
     ctx.pyramid_destroy(p)
     buf.free()
+
+
+def test_autotune_and_cache(ctx, scenes, tmp_path):
+    sh, sc = scenes[0]
+    jobs = [Job(ctx, sh, sc, H.view(yaw=0.5 * k), A.PASS_PLAIN, begin=2000 * k, count=30000) for k in range(4)]
+    old0, old1 = ctx.get_tuning(0), ctx.get_tuning(1)
+    cache = tmp_path / "tune.json"
+    try:
+        t1 = ctx.autotune(1, [j.vj for j in jobs], cache_path=str(cache))
+        t0 = ctx.autotune(0, [j.vj for j in jobs], cache_path=str(cache))
+        assert (t1.block, t1.items) in [(256, 2), (256, 4), (512, 2), (512, 4)]
+        assert t0.grid_per_cu in (0, 1, 2, 4)
+        text = cache.read_text()
+        assert "|k0|b16" in text and "|k1|b16" in text  # 120000 objects -> bucket 16
+        again = ctx.autotune(1, [j.vj for j in jobs], cache_path=str(cache))  # served from the cache
+        assert (again.block, again.items, again.grid_per_cu) == (t1.block, t1.items, t1.grid_per_cu)
+        g = ctx.get_tuning(1)
+        assert (g.block, g.items) == (t1.block, t1.items)
+        ctx.cull_batch([j.vj for j in jobs])
+        ctx.check_kernels()
+        for j in jobs:
+            j.check(ctx, None)
+    finally:
+        ctx.set_tuning(0, old0.block, old0.items, old0.grid_per_cu)
+        ctx.set_tuning(1, old1.block, old1.items, old1.grid_per_cu)
+        for j in jobs:
+            j.free()
