@@ -69,11 +69,13 @@ constexpr int kMaxMips = 16;
 // clamped to the tile count.
 hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
                        uint32_t per_cu, hipStream_t st);
-hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const CompactScratch& s,
-                             uint32_t block, uint32_t items, hipStream_t st);
+// starts_dev: the jobs' tile_start values as a compact ascending array (the batched kernels stage it in LDS)
+hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs, uint32_t total_tiles,
+                             const CompactScratch& s, uint32_t block, uint32_t items, hipStream_t st);
 hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st);
-hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, uint32_t n_jobs, uint32_t total_wgs, uint32_t max_mips,
-                                uint32_t* err, hipStream_t st);
+// starts_dev: the jobs' wg_start values as a compact ascending array (the batched kernels stage it in LDS)
+hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
+                                uint32_t total_wgs, uint32_t* err, hipStream_t st);
 hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
                                 hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t st);

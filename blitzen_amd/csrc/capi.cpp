@@ -150,6 +150,43 @@ bz_status Ctx::cluster_scratch_for(hipStream_t st, StreamScratch* sc, uint32_t t
     return BZ_OK;
 }
 
+// Job tables live in one allocation: cap descriptors, then cap uint32 first-tile (or first-workgroup) indices, which
+// the batched kernels stage in LDS for their job lookup. The host copy is pinned and reused once the previous
+// upload from it has completed.
+template <class T>
+bz_status upload_table(Ctx* ctx, T*& dev, T*& host, uint32_t& cap, hipEvent_t done, const T* jobs,
+                              const uint32_t* starts, uint32_t n, hipStream_t st, const char* what)
+{
+    (void)hipEventSynchronize(done);
+    if (cap < n) {
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
+        dev = nullptr;
+        host = nullptr;
+        cap = 0;
+        const uint32_t c = std::max<uint32_t>(n, 256u);
+        const size_t bytes = (size_t)c * (sizeof(T) + 4);
+        hipError_t e = hipMalloc((void**)&dev, bytes);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&host, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return ctx->hip_fail(e, what);
+        cap = c;
+    }
+    char* hs = reinterpret_cast<char*>(host) + (size_t)cap * sizeof(T);
+    char* ds = reinterpret_cast<char*>(dev) + (size_t)cap * sizeof(T);
+    std::memcpy(host, jobs, (size_t)n * sizeof(T));
+    std::memcpy(hs, starts, (size_t)n * 4);
+    hipError_t e = hipMemcpyAsync(dev, host, (size_t)n * sizeof(T), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, hs, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, what);
+    return BZ_OK;
+}
+
+template <class T>
+const uint32_t* table_starts(const T* dev, uint32_t cap)
+{
+    return reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(dev) + (size_t)cap * sizeof(T));
+}
+
 ViewConsts make_view_consts(const bz_CameraViewData& v)
 {
     ViewConsts c;
@@ -628,7 +665,9 @@ bz_status bz_pyramid_create(bz_ctx* ctx, uint32_t base_w, uint32_t base_h, uint3
         ++tl;
     }
     p->tile_levels = std::min(tl, mips);
-    p->single_pass = pow2 && (p->tile_levels >= mips || (uint64_t)p->tiles_x * p->tiles_y <= 4096u);
+    // single pass: power-of-two extent, and the tile image's first reduction fits the tail workgroup's LDS tile
+    const uint64_t tail_texels = (uint64_t)std::max(1u, p->tiles_x / 2u) * std::max(1u, p->tiles_y / 2u);
+    p->single_pass = pow2 && (p->tile_levels >= mips || tail_texels <= 1024u);
     hipError_t e = hipMalloc((void**)&p->texels, off * 4);
     if (e == hipSuccess) e = hipMemset(p->texels, 0, off * 4);
     const size_t n_tiles = (size_t)p->tiles_x * p->tiles_y;
@@ -747,7 +786,8 @@ bz_status bz_pyramid_build_batch(bz_ctx* ctx, const bz_pyramid_job* jobs, uint32
     if (s) return s;
     s = ctx->upload_pjobs(sc, sp.data(), (uint32_t)sp.size(), st);
     if (s) return s;
-    hipError_t e = launch_pyramid_batch(sc->pjobs_dev, (uint32_t)sp.size(), wg, 0, sc->err + 1, st);
+    hipError_t e = launch_pyramid_batch(sc->pjobs_dev, table_starts(sc->pjobs_dev, sc->pjobs_cap), (uint32_t)sp.size(),
+                                        wg, sc->err + 1, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "pyramid batch launch");
     return BZ_OK;
@@ -836,50 +876,18 @@ bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* str
 namespace bz {
 bz_status Ctx::upload_jobs(StreamScratch* sc, const CullJob* jobs, uint32_t n, hipStream_t st)
 {
-    if (sc->jobs_cap < n) {
-        (void)hipEventSynchronize(sc->upload_done);
-        if (sc->jobs_dev) (void)hipFree(sc->jobs_dev);
-        if (sc->jobs_host) (void)hipHostFree(sc->jobs_host);
-        sc->jobs_dev = nullptr;
-        sc->jobs_host = nullptr;
-        sc->jobs_cap = 0;
-        const uint32_t cap = std::max<uint32_t>(n, 256u);
-        hipError_t e = hipMalloc((void**)&sc->jobs_dev, (size_t)cap * sizeof(CullJob));
-        if (e == hipSuccess) e = hipHostMalloc((void**)&sc->jobs_host, (size_t)cap * sizeof(CullJob), hipHostMallocDefault);
-        if (e != hipSuccess) return hip_fail(e, "job table allocation");
-        sc->jobs_cap = cap;
-    } else {
-        (void)hipEventSynchronize(sc->upload_done);  // the previous upload from jobs_host has completed
-    }
-    std::memcpy(sc->jobs_host, jobs, (size_t)n * sizeof(CullJob));
-    hipError_t e = hipMemcpyAsync(sc->jobs_dev, sc->jobs_host, (size_t)n * sizeof(CullJob), hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return hip_fail(e, "job table upload");
-    return BZ_OK;
+    std::vector<uint32_t> starts(n);
+    for (uint32_t k = 0; k < n; ++k) starts[k] = jobs[k].tile_start;
+    return upload_table(this, sc->jobs_dev, sc->jobs_host, sc->jobs_cap, sc->upload_done, jobs, starts.data(), n, st,
+                        "job table upload");
 }
 
 bz_status Ctx::upload_pjobs(StreamScratch* sc, const PyrBuildJob* jobs, uint32_t n, hipStream_t st)
 {
-    if (sc->pjobs_cap < n) {
-        (void)hipEventSynchronize(sc->upload_done);
-        if (sc->pjobs_dev) (void)hipFree(sc->pjobs_dev);
-        if (sc->pjobs_host) (void)hipHostFree(sc->pjobs_host);
-        sc->pjobs_dev = nullptr;
-        sc->pjobs_host = nullptr;
-        sc->pjobs_cap = 0;
-        const uint32_t cap = std::max<uint32_t>(n, 256u);
-        hipError_t e = hipMalloc((void**)&sc->pjobs_dev, (size_t)cap * sizeof(PyrBuildJob));
-        if (e == hipSuccess)
-            e = hipHostMalloc((void**)&sc->pjobs_host, (size_t)cap * sizeof(PyrBuildJob), hipHostMallocDefault);
-        if (e != hipSuccess) return hip_fail(e, "pyramid job table allocation");
-        sc->pjobs_cap = cap;
-    } else {
-        (void)hipEventSynchronize(sc->upload_done);
-    }
-    std::memcpy(sc->pjobs_host, jobs, (size_t)n * sizeof(PyrBuildJob));
-    hipError_t e =
-        hipMemcpyAsync(sc->pjobs_dev, sc->pjobs_host, (size_t)n * sizeof(PyrBuildJob), hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return hip_fail(e, "pyramid job table upload");
-    return BZ_OK;
+    std::vector<uint32_t> starts(n);
+    for (uint32_t k = 0; k < n; ++k) starts[k] = jobs[k].wg_start;
+    return upload_table(this, sc->pjobs_dev, sc->pjobs_host, sc->pjobs_cap, sc->upload_done, jobs, starts.data(), n,
+                        st, "pyramid job table upload");
 }
 
 bz_status Ctx::make_batch(const bz_view_job* jobs, uint32_t n_jobs, const bz_tuning& t, std::vector<CullJob>& out,
@@ -952,7 +960,8 @@ bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, v
     if (s) return s;
     s = ctx->upload_jobs(sc, cj.data(), (uint32_t)cj.size(), st);
     if (s) return s;
-    e = launch_cull_batch(sc->jobs_dev, (uint32_t)cj.size(), tiles, sc->cs, t.block, t.items, st);
+    e = launch_cull_batch(sc->jobs_dev, table_starts(sc->jobs_dev, sc->jobs_cap), (uint32_t)cj.size(), tiles, sc->cs,
+                          t.block, t.items, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "batched cull launch");
     return BZ_OK;
@@ -1034,14 +1043,25 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (e != hipSuccess) return ctx->hip_fail(e, "graph scratch");
     g->cs.capacity_tiles = cap;
     g->cs.capacity_entries = (uint64_t)cap * t.block * t.items;
+    // job tables (descriptors, then the compact start indices), owned by the graph
     if (!cj.empty()) {
-        e = hipMalloc((void**)&g->jobs_dev, cj.size() * sizeof(CullJob));
+        std::vector<uint32_t> starts(cj.size());
+        for (size_t k = 0; k < cj.size(); ++k) starts[k] = cj[k].tile_start;
+        e = hipMalloc((void**)&g->jobs_dev, cj.size() * (sizeof(CullJob) + 4));
         if (e == hipSuccess) e = hipMemcpy(g->jobs_dev, cj.data(), cj.size() * sizeof(CullJob), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(const_cast<uint32_t*>(table_starts(g->jobs_dev, (uint32_t)cj.size())), starts.data(),
+                          starts.size() * 4, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess && !pj.empty()) {
-        e = hipMalloc((void**)&g->pjobs_dev, pj.size() * sizeof(PyrBuildJob));
+        std::vector<uint32_t> starts(pj.size());
+        for (size_t k = 0; k < pj.size(); ++k) starts[k] = pj[k].wg_start;
+        e = hipMalloc((void**)&g->pjobs_dev, pj.size() * (sizeof(PyrBuildJob) + 4));
         if (e == hipSuccess)
             e = hipMemcpy(g->pjobs_dev, pj.data(), pj.size() * sizeof(PyrBuildJob), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(const_cast<uint32_t*>(table_starts(g->pjobs_dev, (uint32_t)pj.size())), starts.data(),
+                          starts.size() * 4, hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "graph job tables");
     hipStream_t cs = nullptr;
@@ -1049,7 +1069,8 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (e != hipSuccess) return ctx->hip_fail(e, "capture stream");
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess && !pj.empty())
-        e = launch_pyramid_batch(g->pjobs_dev, (uint32_t)pj.size(), wg, 0, g->err + 1, cs);
+        e = launch_pyramid_batch(g->pjobs_dev, table_starts(g->pjobs_dev, (uint32_t)pj.size()), (uint32_t)pj.size(), wg,
+                                 g->err + 1, cs);
     for (auto& m : multi) {
         if (e != hipSuccess) break;
         Pyramid* p = m.first;
@@ -1061,7 +1082,8 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     }
     if (e == hipSuccess) e = zero_counts(empty, cs);
     if (e == hipSuccess && !cj.empty())
-        e = launch_cull_batch(g->jobs_dev, (uint32_t)cj.size(), tiles, g->cs, t.block, t.items, cs);
+        e = launch_cull_batch(g->jobs_dev, table_starts(g->jobs_dev, (uint32_t)cj.size()), (uint32_t)cj.size(), tiles,
+                              g->cs, t.block, t.items, cs);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     (void)hipStreamDestroy(cs);

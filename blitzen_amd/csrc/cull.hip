@@ -411,28 +411,40 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
     }
 }
 
-__device__ __forceinline__ uint32_t find_job(const CullJob* jobs, uint32_t n_jobs, uint32_t t)
+// Last entry of the ascending table `starts` that is <= t.
+__device__ __forceinline__ uint32_t find_job(const uint32_t* starts, uint32_t n_jobs, uint32_t t)
 {
-    uint32_t lo = 0, hi = n_jobs - 1;  // last job with tile_start <= t
+    uint32_t lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_start <= t) lo = mid;
+        if (starts[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
     return lo;
 }
 
-// Batched decide: one launch over the (job, tile) work list of many views; workgroup b takes global tile b.
+constexpr uint32_t kLdsJobs = 1024;  // batched: job start table staged in LDS up to this many jobs
+
+// Batched decide: one launch over the (job, tile) work list of many views; workgroup b takes global tile b. The
+// compact table of the jobs' first tiles is staged in LDS (one coalesced read) for the job lookup.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void decide_batch_kernel(const CullJob* __restrict__ jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(BLOCK) void decide_batch_kernel(const CullJob* __restrict__ jobs,
+                                                             const uint32_t* __restrict__ starts, uint32_t n_jobs,
                                                              uint32_t* counts, uint2* staging)
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
     __shared__ CullJob sjob;
+    __shared__ uint32_t s_start[kLdsJobs];
     const uint32_t t = blockIdx.x;
-    if (threadIdx.x == 0) sm.job = find_job(jobs, n_jobs, t);
-    __syncthreads();
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[sm.job]);
+    uint32_t jb;
+    if (n_jobs <= kLdsJobs) {
+        for (uint32_t k = threadIdx.x; k < n_jobs; k += BLOCK) s_start[k] = starts[k];
+        __syncthreads();
+        jb = find_job(s_start, n_jobs, t);
+    } else {
+        jb = find_job(starts, n_jobs, t);
+    }
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[jb]);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&sjob);
     for (uint32_t w = threadIdx.x; w < sizeof(CullJob) / 4; w += BLOCK) dst[w] = src[w];
     __syncthreads();
@@ -585,7 +597,8 @@ __global__ __launch_bounds__(kEmitBlock) void emit_kernel(CullJob J, uint32_t ti
 }
 
 // Batched emit: workgroup b covers global tiles [b*kEmitTiles, ...), split at job boundaries.
-__global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* __restrict__ jobs, uint32_t n_jobs,
+__global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* __restrict__ jobs,
+                                                                const uint32_t* __restrict__ starts, uint32_t n_jobs,
                                                                 uint32_t total_tiles, uint32_t tile_size,
                                                                 const uint32_t* counts, const uint2* staging)
 {
@@ -594,7 +607,7 @@ __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* _
     const uint32_t t_end = t + kEmitTiles < total_tiles ? t + kEmitTiles : total_tiles;
     while (t < t_end) {
         __syncthreads();
-        if (threadIdx.x == 0) sm.job = find_job(jobs, n_jobs, t);
+        if (threadIdx.x == 0) sm.job = find_job(starts, n_jobs, t);
         __syncthreads();
         const CullJob& J = jobs[sm.job];
         const uint32_t job_end = J.tile_start + J.n_tiles;
@@ -660,15 +673,15 @@ hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n
 }
 
 template <int BLOCK, int ITEMS>
-hipError_t launch_cull_batch_t(const CullJob* jobs, uint32_t n_jobs, uint32_t total_tiles, const CompactScratch& s,
-                               hipStream_t st)
+hipError_t launch_cull_batch_t(const CullJob* jobs, const uint32_t* starts, uint32_t n_jobs, uint32_t total_tiles,
+                               const CompactScratch& s, hipStream_t st)
 {
-    hipLaunchKernelGGL((decide_batch_kernel<BLOCK, ITEMS>), dim3(total_tiles), dim3(BLOCK), 0, st, jobs, n_jobs,
-                       s.counts, s.staging);
+    hipLaunchKernelGGL((decide_batch_kernel<BLOCK, ITEMS>), dim3(total_tiles), dim3(BLOCK), 0, st, jobs, starts,
+                       n_jobs, s.counts, s.staging);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(emit_batch_kernel, dim3((total_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st,
-                       jobs, n_jobs, total_tiles, (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts,
+                       jobs, starts, n_jobs, total_tiles, (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts,
                        (const uint2*)s.staging);
     return hipGetLastError();
 }
@@ -698,12 +711,12 @@ hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, u
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_cull_batch(const CullJob* jobs_dev, uint32_t n_jobs, uint32_t total_tiles, const CompactScratch& s,
-                             uint32_t block, uint32_t items, hipStream_t st)
+hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs, uint32_t total_tiles,
+                             const CompactScratch& s, uint32_t block, uint32_t items, hipStream_t st)
 {
     if (n_jobs == 0 || total_tiles == 0) return hipSuccess;
 #define X(B, I) \
-    if (block == B && items == I) return launch_cull_batch_t<B, I>(jobs_dev, n_jobs, total_tiles, s, st);
+    if (block == B && items == I) return launch_cull_batch_t<B, I>(jobs_dev, starts_dev, n_jobs, total_tiles, s, st);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;

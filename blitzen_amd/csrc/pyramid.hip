@@ -18,18 +18,20 @@ namespace {
 
 constexpr uint32_t kPyrBlock = 256;
 constexpr uint32_t kTile = 32;
-constexpr uint32_t kTailMax = 4096;  // texels of the per-tile image the last workgroup reduces in LDS
 constexpr unsigned long long kTailFlag = 1ull << 63;
 constexpr uint32_t kTailSpinLimit = 1u << 20;
+constexpr uint32_t kLdsJobs = 1024;  // batched: job start table staged in LDS up to this many jobs
 
 __device__ __forceinline__ float min4(float a, float b, float c, float d) { return glsl_min(glsl_min(a, b), glsl_min(c, d)); }
 
-// Reduce src (sw x sh, LDS) to dst (dw x dh = max(1, s/2)) exactly 2:1 with clamp; optionally write to global.
+// Reduce src (sw x sh, LDS) to dst (dw x dh = max(1, s/2)) exactly 2:1 with clamp, and write it to global.
+// All extents are powers of two (single-pass pyramids only).
 __device__ __forceinline__ void reduce_lds(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw,
                                            uint32_t dh, float* gdst, uint32_t gpitch, uint32_t gx, uint32_t gy)
 {
+    const uint32_t dshift = (uint32_t)__builtin_ctz(dw);
     for (uint32_t t = threadIdx.x; t < dw * dh; t += kPyrBlock) {
-        const uint32_t lx = t % dw, ly = t / dw;
+        const uint32_t lx = t & (dw - 1u), ly = t >> dshift;
         const uint32_t x0 = 2u * lx, y0 = 2u * ly;
         const uint32_t x1 = sw > 1u ? x0 + 1u : x0;
         const uint32_t y1 = sh > 1u ? y0 + 1u : y0;
@@ -39,24 +41,87 @@ __device__ __forceinline__ void reduce_lds(const float* src, uint32_t sw, uint32
     }
 }
 
-__device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, float* bufB, uint32_t* s_last,
-                             uint32_t* err)
+// Reads one per-tile word for the tail: the owner's own value, or the flagged word another workgroup stored
+// (cleared after reading, so the next build starts from zero). The wait is bounded; giving up sets *err.
+__device__ __forceinline__ float tail_word(const PyrBuildJob& J, uint32_t idx, uint32_t own_tile, float own,
+                                           uint32_t* err)
 {
+    if (idx == own_tile) return own;
+    unsigned long long w = __hip_atomic_load(&J.tail[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (!(w & kTailFlag)) {
+        if (++spins > kTailSpinLimit) {
+            atomicExch(err, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w = __hip_atomic_load(&J.tail[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __hip_atomic_store(&J.tail[idx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float((uint32_t)w);
+}
+
+constexpr uint32_t kTileTexels = kTile * kTile;
+constexpr uint32_t kPerThread = kTileTexels / kPyrBlock;
+
+// One workgroup's tile of a single-pass (power-of-two) pyramid. bufA holds kTileTexels floats, bufB half that.
+__device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, float* bufB, uint32_t* err)
+{
+    const uint32_t tid = threadIdx.x;
     const uint32_t tx = tile % J.tiles_x, ty = tile / J.tiles_x;
-    const uint32_t tw = J.tile_w, th = J.tile_h;
-    // level 0 from the depth buffer (general footprint: the depth buffer need not be 2:1 of the base)
-    const float fbw = (float)J.base_w, fbh = (float)J.base_h;
-    for (uint32_t t = threadIdx.x; t < tw * th; t += kPyrBlock) {
-        const uint32_t lx = t % tw, ly = t / tw;
-        const uint32_t x = tx * tw + lx, y = ty * th + ly;
-        const float u = ((float)x + 0.5f) / fbw;
-        const float v = ((float)y + 0.5f) / fbh;
-        const Footprint f = footprint(u, v, J.w, J.h);
-        const float d = footprint_min(J.depth, J.w, f);
-        bufA[t] = d;
-        J.texels[J.level_offset[0] + (size_t)y * J.base_w + x] = d;
+    const uint32_t tw = J.tile_w, th = J.tile_h, n0 = tw * th;
+    const uint32_t tshift = (uint32_t)__builtin_ctz(tw);
+    float* lvl0 = J.texels + J.level_offset[0];
+    // ---- level 0 from the depth buffer; every load of a thread is issued before the first min
+    if (J.w == 2u * J.base_w && J.h == 2u * J.base_h && ((uintptr_t)J.depth & 7u) == 0) {
+        // exactly 2:1: the footprint of texel (x, y) is depth {2x, 2x+1} x {2y, 2y+1} (frac(tx) = 0.5 on both axes)
+        float2 r0[kPerThread], r1[kPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t t0 = tid + k * kPyrBlock, t = t0 < n0 ? t0 : 0u;
+            const uint32_t x = tx * tw + (t & (tw - 1u)), y = ty * th + (t >> tshift);
+            const float2* row = reinterpret_cast<const float2*>(J.depth + (size_t)(2u * y) * J.w) + x;
+            r0[k] = row[0];
+            r1[k] = row[J.w / 2u];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t t = tid + k * kPyrBlock;
+            if (t < n0) {
+                const uint32_t x = tx * tw + (t & (tw - 1u)), y = ty * th + (t >> tshift);
+                const float d = min4(r0[k].x, r0[k].y, r1[k].x, r1[k].y);
+                bufA[t] = d;
+                lvl0[(size_t)y * J.base_w + x] = d;
+            }
+        }
+    } else {
+        // general ratio: the footprint rule; the base extent is a power of two, so (x + 0.5) / W equals
+        // (x + 0.5) * (1 / W) exactly
+        const float rbw = 1.0f / (float)J.base_w, rbh = 1.0f / (float)J.base_h;
+        float d00[kPerThread], d01[kPerThread], d10[kPerThread], d11[kPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t t0 = tid + k * kPyrBlock, t = t0 < n0 ? t0 : 0u;
+            const uint32_t x = tx * tw + (t & (tw - 1u)), y = ty * th + (t >> tshift);
+            const Footprint f = footprint(((float)x + 0.5f) * rbw, ((float)y + 0.5f) * rbh, J.w, J.h);
+            d00[k] = J.depth[(size_t)f.y0 * J.w + f.x0];
+            d01[k] = J.depth[(size_t)f.y0 * J.w + f.x1];
+            d10[k] = J.depth[(size_t)f.y1 * J.w + f.x0];
+            d11[k] = J.depth[(size_t)f.y1 * J.w + f.x1];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPerThread; ++k) {
+            const uint32_t t = tid + k * kPyrBlock;
+            if (t < n0) {
+                const uint32_t x = tx * tw + (t & (tw - 1u)), y = ty * th + (t >> tshift);
+                const float d = min4(d00[k], d01[k], d10[k], d11[k]);
+                bufA[t] = d;
+                lvl0[(size_t)y * J.base_w + x] = d;
+            }
+        }
     }
     __syncthreads();
+    // ---- levels inside the tile, in LDS
     uint32_t cw = tw, ch = th;
     float* src = bufA;
     float* dst = bufB;
@@ -77,39 +142,39 @@ __device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, f
     // workgroup) nor a shared completion counter (a serialised atomic per workgroup) is needed.
     const uint32_t n_tiles = J.tiles_x * J.tiles_y;
     if (tile + 1u < n_tiles) {
-        if (threadIdx.x == 0)
+        if (tid == 0)
             __hip_atomic_store(&J.tail[tile], kTailFlag | __float_as_uint(src[0]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    (void)s_last;
+    const float own = src[0];
+    __syncthreads();  // every thread has read `own` before bufA is rewritten
+    // the tile image (one texel per tile, level tile_levels - 1) is reduced to the next level while it is collected
     const uint32_t L0 = J.tile_levels - 1u;
     cw = level_dim(J.base_w, L0);
     ch = level_dim(J.base_h, L0);
-    const float own = src[0];
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < cw * ch; t += kPyrBlock) {
-        if (t == tile) {
-            bufA[t] = own;
-            continue;
+    {
+        const uint32_t nw = cw > 1u ? cw >> 1 : 1u, nh = ch > 1u ? ch >> 1 : 1u;
+        const uint32_t nshift = (uint32_t)__builtin_ctz(nw);
+        float* g = J.texels + J.level_offset[L0 + 1u];
+        for (uint32_t t = tid; t < nw * nh; t += kPyrBlock) {
+            const uint32_t x0 = 2u * (t & (nw - 1u)), y0 = 2u * (t >> nshift);
+            const bool hx = cw > 1u, hy = ch > 1u;  // a second column / row exists (else the clamp repeats the first)
+            const float v00 = tail_word(J, y0 * cw + x0, tile, own, err);
+            const float v01 = hx ? tail_word(J, y0 * cw + x0 + 1u, tile, own, err) : v00;
+            const float v10 = hy ? tail_word(J, (y0 + 1u) * cw + x0, tile, own, err) : v00;
+            const float v11 = (hx && hy) ? tail_word(J, (y0 + 1u) * cw + x0 + 1u, tile, own, err) : (hx ? v01 : v10);
+            const float d = min4(v00, v01, v10, v11);
+            bufA[t] = d;
+            g[t] = d;
         }
-        unsigned long long w = __hip_atomic_load(&J.tail[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t spins = 0;
-        while (!(w & kTailFlag)) {
-            if (++spins > kTailSpinLimit) {
-                atomicExch(err, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            w = __hip_atomic_load(&J.tail[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        bufA[t] = __uint_as_float((uint32_t)w);
-        __hip_atomic_store(&J.tail[t], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cw = nw;
+        ch = nh;
     }
     __syncthreads();
     src = bufA;
     dst = bufB;
-    for (uint32_t L = L0 + 1u; L < J.mips; ++L) {
+    for (uint32_t L = L0 + 2u; L < J.mips; ++L) {
         const uint32_t nw = cw > 1u ? cw >> 1 : 1u, nh = ch > 1u ? ch >> 1 : 1u;
         reduce_lds(src, cw, ch, dst, nw, nh, J.texels + J.level_offset[L], nw, 0u, 0u);
         __syncthreads();
@@ -123,31 +188,46 @@ __device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, f
 
 __global__ __launch_bounds__(kPyrBlock) void pyramid_kernel(PyrBuildJob J, uint32_t* err)
 {
-    __shared__ float bufA[kTailMax];
-    __shared__ float bufB[kTailMax / 2];
-    __shared__ uint32_t s_last;
-    pyramid_tile(J, blockIdx.x, bufA, bufB, &s_last, err);
+    __shared__ float bufA[kTileTexels];
+    __shared__ float bufB[kTileTexels / 2];
+    pyramid_tile(J, blockIdx.x, bufA, bufB, err);
 }
 
-__global__ __launch_bounds__(kPyrBlock) void pyramid_batch_kernel(const PyrBuildJob* __restrict__ jobs, uint32_t n_jobs,
+// Last entry of the ascending table `starts` that is <= x (searched in LDS when staged there).
+__device__ __forceinline__ uint32_t last_le(const uint32_t* starts, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (starts[mid] <= x) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// Many pyramids in one launch: workgroup b builds tile b - wg_start of the job whose range holds b. The compact
+// table of first workgroups is staged in LDS (one coalesced read) and the job descriptor is copied to LDS.
+__global__ __launch_bounds__(kPyrBlock) void pyramid_batch_kernel(const PyrBuildJob* __restrict__ jobs,
+                                                                  const uint32_t* __restrict__ starts, uint32_t n_jobs,
                                                                   uint32_t* err)
 {
-    __shared__ float bufA[kTailMax];
-    __shared__ float bufB[kTailMax / 2];
-    __shared__ uint32_t s_last;
-    __shared__ uint32_t s_job;
-    if (threadIdx.x == 0) {
-        uint32_t lo = 0, hi = n_jobs - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].wg_start <= blockIdx.x) lo = mid;
-            else hi = mid - 1;
-        }
-        s_job = lo;
+    __shared__ float bufA[kTileTexels];
+    __shared__ float bufB[kTileTexels / 2];
+    __shared__ uint32_t s_start[kLdsJobs];
+    __shared__ PyrBuildJob sJ;
+    uint32_t j;
+    if (n_jobs <= kLdsJobs) {
+        for (uint32_t k = threadIdx.x; k < n_jobs; k += kPyrBlock) s_start[k] = starts[k];
+        __syncthreads();
+        j = last_le(s_start, n_jobs, blockIdx.x);
+    } else {
+        j = last_le(starts, n_jobs, blockIdx.x);
     }
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[j]);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&sJ);
+    for (uint32_t w = threadIdx.x; w < sizeof(PyrBuildJob) / 4; w += kPyrBlock) dst[w] = src[w];
     __syncthreads();
-    const PyrBuildJob J = jobs[s_job];
-    pyramid_tile(J, blockIdx.x - J.wg_start, bufA, bufB, &s_last, err);
+    pyramid_tile(sJ, blockIdx.x - sJ.wg_start, bufA, bufB, err);
 }
 
 // General per-level footprint reduction (any sizes).
@@ -209,12 +289,12 @@ hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, uint32_t n_jobs, uint32_t total_wgs, uint32_t max_mips,
-                                uint32_t* err, hipStream_t st)
+hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
+                                uint32_t total_wgs, uint32_t* err, hipStream_t st)
 {
-    (void)max_mips;
     if (n_jobs == 0 || total_wgs == 0) return hipSuccess;
-    hipLaunchKernelGGL(pyramid_batch_kernel, dim3(total_wgs), dim3(kPyrBlock), 0, st, jobs_dev, n_jobs, err);
+    hipLaunchKernelGGL(pyramid_batch_kernel, dim3(total_wgs), dim3(kPyrBlock), 0, st, jobs_dev, starts_dev, n_jobs,
+                       err);
     return hipGetLastError();
 }
 

@@ -57,7 +57,8 @@ def test_plain_c0_bit_exact(ctx):
 
 @pytest.mark.parametrize("dims", [(1920, 1080, 1024, 512, 11), (1920, 1080, 1024, 1024, 10), (512, 512, 256, 256, 8),
                                   (2048, 1024, 1024, 512, 10), (1920, 1080, 960, 540, 10), (37, 23, 16, 8, 5),
-                                  (4096, 2048, 2048, 1024, 12)])
+                                  (4096, 2048, 2048, 1024, 12), (2048, 64, 1024, 32, 11), (64, 4096, 32, 2048, 12),
+                                  (3000, 100, 2048, 64, 12)])
 def test_pyramid_bit_exact(ctx, dims):
     w, h, bw, bh, mips = dims
     depth = H.c1_depth(w=w, h=h, occluders=256)
