@@ -153,6 +153,8 @@ SIGNATURES = {
     "bz_server_start": (_I32, [_VP, C.POINTER(ServerCfg)]),
     "bz_server_stop": (_I32, [_VP]),
     "bz_server_submit": (_I32, [_VP, C.POINTER(Request), C.POINTER(_U64)]),
+    "bz_server_submit_schedule": (_I32, [_VP, C.POINTER(Request), C.POINTER(_U64), _U32, C.POINTER(_U64),
+                                         C.POINTER(C.c_int32)]),
     "bz_server_poll": (_I32, [_VP, _U64, C.POINTER(Result)]),
     "bz_server_wait": (_I32, [_VP, _U64, _U64, C.POINTER(Result)]),
     "bz_server_fetch": (_I32, [_VP, _U64, _VP, _U32, C.POINTER(_U32)]),

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -40,16 +42,27 @@ struct Scene {
     ~Scene();
 };
 
-struct Pyramid {
+// Level layout and single-pass tiling of a min-depth pyramid (no storage).
+struct PyrPlan {
     uint32_t base_w = 0, base_h = 0, mips = 0;
     uint32_t level_offset[kMaxMips] = {};
     uint64_t n_texels = 0;
-    float* texels = nullptr;
-    unsigned long long* tail = nullptr;
     uint32_t tile_w = 0, tile_h = 0, tiles_x = 0, tiles_y = 0, tile_levels = 0;
     bool single_pass = false;
-    PyramidDesc desc() const;
-    PyrBuildJob build_job(const float* depth, uint32_t w, uint32_t h) const;
+    PyramidDesc desc(const float* texels) const;
+    PyrBuildJob build_job(const float* depth, uint32_t w, uint32_t h, float* texels, unsigned long long* tail) const;
+};
+// Fills `out` for a base_w x base_h pyramid of `mips` levels; false if it holds more than 2^32 - 1 texels.
+bool plan_pyramid(uint32_t base_w, uint32_t base_h, uint32_t mips, PyrPlan* out);
+
+struct Pyramid : PyrPlan {
+    float* texels = nullptr;
+    unsigned long long* tail = nullptr;  // [tiles_x * tiles_y] flagged per-tile words of the single-pass build
+    PyramidDesc desc() const { return PyrPlan::desc(texels); }
+    PyrBuildJob build_job(const float* depth, uint32_t w, uint32_t h) const
+    {
+        return PyrPlan::build_job(depth, w, h, texels, tail);
+    }
     ~Pyramid();
 };
 
@@ -116,6 +129,44 @@ struct Ctx {
 };
 
 ViewConsts make_view_consts(const bz_CameraViewData& v);
+
+// Job tables live in one allocation: cap descriptors, then cap uint32 first-tile (or first-workgroup) indices, which
+// the batched kernels stage in LDS for their job lookup. The host copy is pinned and reused once the previous
+// upload from it has completed.
+template <class T>
+inline bz_status upload_table(Ctx* ctx, T*& dev, T*& host, uint32_t& cap, hipEvent_t done, const T* jobs,
+                              const uint32_t* starts, uint32_t n, hipStream_t st, const char* what)
+{
+    (void)hipEventSynchronize(done);
+    if (cap < n) {
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
+        dev = nullptr;
+        host = nullptr;
+        cap = 0;
+        const uint32_t c = std::max<uint32_t>(n, 256u);
+        const size_t bytes = (size_t)c * (sizeof(T) + 4);
+        hipError_t e = hipMalloc((void**)&dev, bytes);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&host, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return ctx->hip_fail(e, what);
+        cap = c;
+    }
+    char* hs = reinterpret_cast<char*>(host) + (size_t)cap * sizeof(T);
+    char* ds = reinterpret_cast<char*>(dev) + (size_t)cap * sizeof(T);
+    std::memcpy(host, jobs, (size_t)n * sizeof(T));
+    std::memcpy(hs, starts, (size_t)n * 4);
+    hipError_t e = hipMemcpyAsync(dev, host, (size_t)n * sizeof(T), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, hs, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, what);
+    return BZ_OK;
+}
+
+template <class T>
+inline const uint32_t* table_starts(const T* dev, uint32_t cap)
+{
+    return reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(dev) + (size_t)cap * sizeof(T));
+}
+
 uint32_t tiles_for(uint32_t n, uint32_t block, uint32_t items);
 
 }  // namespace bz

@@ -85,6 +85,21 @@ hipError_t launch_synth_depth(uint64_t seed, uint32_t w, uint32_t h, uint32_t oc
                               float* depth, hipStream_t st);
 
 // Cluster two-level cull
+// One synthetic depth buffer of a batched raster launch (one workgroup per 64 x 64 pixel tile). The first
+// workgroup of a job also clears `n_tail` words at `tail_zero` (the flagged words of a pyramid built next).
+struct SynthJob {
+    uint64_t seed;
+    float* depth;
+    unsigned long long* tail_zero;
+    uint32_t w, h, occluders, tiles_x;
+    float p11, znear;
+    uint32_t n_tail, wg_start;
+};
+hipError_t launch_synth_depth_batch(const SynthJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
+                                    uint32_t total_wgs, hipStream_t st);
+constexpr uint32_t kSynthTile = 64;       // raster tile edge
+constexpr uint32_t kSynthMaxRects = 4096; // occluder rectangles per depth buffer
+
 // Cluster pass of the two-level cluster cull (the object pass is a CullJob run by launch_cluster_objects).
 struct ClusterJob {
     SceneDev scene;

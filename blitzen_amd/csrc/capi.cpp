@@ -150,43 +150,6 @@ bz_status Ctx::cluster_scratch_for(hipStream_t st, StreamScratch* sc, uint32_t t
     return BZ_OK;
 }
 
-// Job tables live in one allocation: cap descriptors, then cap uint32 first-tile (or first-workgroup) indices, which
-// the batched kernels stage in LDS for their job lookup. The host copy is pinned and reused once the previous
-// upload from it has completed.
-template <class T>
-bz_status upload_table(Ctx* ctx, T*& dev, T*& host, uint32_t& cap, hipEvent_t done, const T* jobs,
-                              const uint32_t* starts, uint32_t n, hipStream_t st, const char* what)
-{
-    (void)hipEventSynchronize(done);
-    if (cap < n) {
-        if (dev) (void)hipFree(dev);
-        if (host) (void)hipHostFree(host);
-        dev = nullptr;
-        host = nullptr;
-        cap = 0;
-        const uint32_t c = std::max<uint32_t>(n, 256u);
-        const size_t bytes = (size_t)c * (sizeof(T) + 4);
-        hipError_t e = hipMalloc((void**)&dev, bytes);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&host, bytes, hipHostMallocDefault);
-        if (e != hipSuccess) return ctx->hip_fail(e, what);
-        cap = c;
-    }
-    char* hs = reinterpret_cast<char*>(host) + (size_t)cap * sizeof(T);
-    char* ds = reinterpret_cast<char*>(dev) + (size_t)cap * sizeof(T);
-    std::memcpy(host, jobs, (size_t)n * sizeof(T));
-    std::memcpy(hs, starts, (size_t)n * 4);
-    hipError_t e = hipMemcpyAsync(dev, host, (size_t)n * sizeof(T), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(ds, hs, (size_t)n * 4, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return ctx->hip_fail(e, what);
-    return BZ_OK;
-}
-
-template <class T>
-const uint32_t* table_starts(const T* dev, uint32_t cap)
-{
-    return reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(dev) + (size_t)cap * sizeof(T));
-}
-
 ViewConsts make_view_consts(const bz_CameraViewData& v)
 {
     ViewConsts c;
@@ -226,7 +189,7 @@ SceneDev Scene::dev() const
     return d;
 }
 
-PyramidDesc Pyramid::desc() const
+PyramidDesc PyrPlan::desc(const float* texels) const
 {
     PyramidDesc p;
     std::memset(&p, 0, sizeof p);
@@ -236,6 +199,36 @@ PyramidDesc Pyramid::desc() const
     p.mips = mips;
     for (uint32_t i = 0; i < mips && i < (uint32_t)kMaxMips; ++i) p.level_offset[i] = level_offset[i];
     return p;
+}
+
+bool plan_pyramid(uint32_t base_w, uint32_t base_h, uint32_t mips, PyrPlan* p)
+{
+    *p = PyrPlan{};
+    p->base_w = base_w;
+    p->base_h = base_h;
+    p->mips = mips;
+    uint64_t off = 0;
+    for (uint32_t L = 0; L < mips; ++L) {
+        p->level_offset[L] = (uint32_t)off;
+        off += (uint64_t)level_dim(base_w, L) * level_dim(base_h, L);
+    }
+    if (off > 0xffffffffull) return false;
+    p->n_texels = off;
+    const bool pow2 = (base_w & (base_w - 1)) == 0 && (base_h & (base_h - 1)) == 0;
+    p->tile_w = std::min<uint32_t>(32u, base_w);
+    p->tile_h = std::min<uint32_t>(32u, base_h);
+    p->tiles_x = base_w / p->tile_w;
+    p->tiles_y = base_h / p->tile_h;
+    uint32_t tl = 1, m = std::max(p->tile_w, p->tile_h);
+    while (m > 1) {
+        m >>= 1;
+        ++tl;
+    }
+    p->tile_levels = std::min(tl, mips);
+    // single pass: power-of-two extent, and the tile image's first reduction fits the tail workgroup's LDS tile
+    const uint64_t tail_texels = (uint64_t)std::max(1u, p->tiles_x / 2u) * std::max(1u, p->tiles_y / 2u);
+    p->single_pass = pow2 && (p->tile_levels >= mips || tail_texels <= 1024u);
+    return true;
 }
 
 uint32_t tiles_for(uint32_t n, uint32_t block, uint32_t items)
@@ -498,6 +491,7 @@ bz_status bz_scene_destroy(bz_ctx* ctx, bz_scene scene)
     if (!ctx) return BZ_E_INVALID;
     auto it = ctx->scenes.find(scene);
     if (it == ctx->scenes.end()) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (ctx->server) return ctx->fail(BZ_E_STATE, "scenes cannot be destroyed while the server runs");
     (void)hipDeviceSynchronize();
     ctx->scenes.erase(it);
     return BZ_OK;
@@ -644,30 +638,8 @@ bz_status bz_pyramid_create(bz_ctx* ctx, uint32_t base_w, uint32_t base_h, uint3
         return ctx->fail(BZ_E_INVALID, "pyramid extent must be non-zero with 1..16 mips");
     std::unique_ptr<Pyramid> p(new (std::nothrow) Pyramid());
     if (!p) return BZ_E_OOM;
-    p->base_w = base_w;
-    p->base_h = base_h;
-    p->mips = mips;
-    uint64_t off = 0;
-    for (uint32_t L = 0; L < mips; ++L) {
-        p->level_offset[L] = (uint32_t)off;
-        off += (uint64_t)level_dim(base_w, L) * level_dim(base_h, L);
-    }
-    if (off > 0xffffffffull) return ctx->fail(BZ_E_INVALID, "pyramid too large");
-    p->n_texels = off;
-    const bool pow2 = (base_w & (base_w - 1)) == 0 && (base_h & (base_h - 1)) == 0;
-    p->tile_w = std::min<uint32_t>(32u, base_w);
-    p->tile_h = std::min<uint32_t>(32u, base_h);
-    p->tiles_x = base_w / p->tile_w;
-    p->tiles_y = base_h / p->tile_h;
-    uint32_t tl = 1, m = std::max(p->tile_w, p->tile_h);
-    while (m > 1) {
-        m >>= 1;
-        ++tl;
-    }
-    p->tile_levels = std::min(tl, mips);
-    // single pass: power-of-two extent, and the tile image's first reduction fits the tail workgroup's LDS tile
-    const uint64_t tail_texels = (uint64_t)std::max(1u, p->tiles_x / 2u) * std::max(1u, p->tiles_y / 2u);
-    p->single_pass = pow2 && (p->tile_levels >= mips || tail_texels <= 1024u);
+    if (!plan_pyramid(base_w, base_h, mips, p.get())) return ctx->fail(BZ_E_INVALID, "pyramid too large");
+    const uint64_t off = p->n_texels;
     hipError_t e = hipMalloc((void**)&p->texels, off * 4);
     if (e == hipSuccess) e = hipMemset(p->texels, 0, off * 4);
     const size_t n_tiles = (size_t)p->tiles_x * p->tiles_y;
@@ -703,7 +675,8 @@ bz_status bz_pyramid_storage(bz_ctx* ctx, bz_pyramid pyramid, float** texels, ui
 }
 
 namespace bz {
-PyrBuildJob Pyramid::build_job(const float* depth, uint32_t w, uint32_t h) const
+PyrBuildJob PyrPlan::build_job(const float* depth, uint32_t w, uint32_t h, float* texels,
+                               unsigned long long* tail) const
 {
     PyrBuildJob j;
     std::memset(&j, 0, sizeof j);

@@ -243,42 +243,93 @@ __global__ void pyramid_level_kernel(const float* __restrict__ src, uint32_t sw,
     dst[(size_t)y * dw + x] = footprint_min(src, sw, f);
 }
 
-// Synthetic depth: one workgroup per 64 x 64 pixel tile. The workgroup regenerates every rectangle from the seed,
-// keeps those overlapping its tile in LDS, then each pixel takes the max (nearest, reverse-z) of the ground plane
-// and its covering rectangles. max is exact, so the order of the LDS list does not matter.
-constexpr uint32_t kRasterTile = 64;
-constexpr uint32_t kMaxRects = 4096;
+// Synthetic depth: one workgroup per 64 x 64 pixel tile. The workgroup regenerates the rectangles from the seed in
+// chunks, keeps those overlapping its tile in LDS, and each pixel (16 per thread, in registers) takes the max
+// (nearest, reverse-z) of the ground plane and its covering rectangles. max is exact, so the order of the LDS list
+// does not matter.
+constexpr uint32_t kRasterTile = kSynthTile;
+constexpr uint32_t kMaxRects = kSynthMaxRects;
+constexpr uint32_t kRectChunk = 512;
+constexpr uint32_t kRasterPix = kRasterTile * kRasterTile / 256;
+
+__device__ void synth_depth_tile(uint64_t seed, uint32_t W, uint32_t H, uint32_t n_rects, float p11, float znear,
+                                 float* __restrict__ depth, uint32_t bx, uint32_t by, int4* s_rect, float* s_depth,
+                                 uint32_t* s_n)
+{
+    const int32_t tx0 = (int32_t)(bx * kRasterTile), ty0 = (int32_t)(by * kRasterTile);
+    const int32_t tx1 = tx0 + (int32_t)kRasterTile, ty1 = ty0 + (int32_t)kRasterTile;
+    float d[kRasterPix];
+#pragma unroll
+    for (uint32_t k = 0; k < kRasterPix; ++k) {
+        const uint32_t t = threadIdx.x + k * 256u;
+        const int32_t y = ty0 + (int32_t)(t / kRasterTile);
+        d[k] = synth_plane_depth((uint32_t)(y < (int32_t)H ? y : 0), H, p11, znear);
+    }
+    for (uint32_t r0 = 0; r0 < n_rects; r0 += kRectChunk) {
+        if (threadIdx.x == 0) *s_n = 0;
+        __syncthreads();
+        const uint32_t r1 = r0 + kRectChunk < n_rects ? r0 + kRectChunk : n_rects;
+        for (uint32_t r = r0 + threadIdx.x; r < r1; r += 256) {
+            const SynthRect sr = synth_rect(seed, W, H, r);
+            if (sr.x0 < tx1 && sr.x1 > tx0 && sr.y0 < ty1 && sr.y1 > ty0) {
+                const uint32_t slot = atomicAdd(s_n, 1u);
+                s_rect[slot] = make_int4(sr.x0, sr.y0, sr.x1, sr.y1);
+                s_depth[slot] = sr.depth;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = *s_n;
+        for (uint32_t j = 0; j < n; ++j) {
+            const int4 rc = s_rect[j];
+            const float rd = s_depth[j];
+#pragma unroll
+            for (uint32_t k = 0; k < kRasterPix; ++k) {
+                const uint32_t t = threadIdx.x + k * 256u;
+                const int32_t x = tx0 + (int32_t)(t % kRasterTile), y = ty0 + (int32_t)(t / kRasterTile);
+                if (x >= rc.x && x < rc.z && y >= rc.y && y < rc.w) d[k] = glsl_max(d[k], rd);
+            }
+        }
+        __syncthreads();  // the list is rewritten by the next chunk
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRasterPix; ++k) {
+        const uint32_t t = threadIdx.x + k * 256u;
+        const int32_t x = tx0 + (int32_t)(t % kRasterTile), y = ty0 + (int32_t)(t / kRasterTile);
+        if (x < (int32_t)W && y < (int32_t)H) depth[(size_t)y * W + x] = d[k];
+    }
+}
 
 __global__ __launch_bounds__(256) void synth_depth_kernel(uint64_t seed, uint32_t W, uint32_t H, uint32_t n_rects,
                                                           float p11, float znear, float* __restrict__ depth)
 {
-    __shared__ int4 s_rect[kMaxRects];
-    __shared__ float s_depth[kMaxRects];
+    __shared__ int4 s_rect[kRectChunk];
+    __shared__ float s_depth[kRectChunk];
     __shared__ uint32_t s_n;
-    const int32_t tx0 = (int32_t)(blockIdx.x * kRasterTile), ty0 = (int32_t)(blockIdx.y * kRasterTile);
-    const int32_t tx1 = tx0 + (int32_t)kRasterTile, ty1 = ty0 + (int32_t)kRasterTile;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    for (uint32_t r = threadIdx.x; r < n_rects; r += 256) {
-        const SynthRect s = synth_rect(seed, W, H, r);
-        if (s.x0 < tx1 && s.x1 > tx0 && s.y0 < ty1 && s.y1 > ty0) {
-            const uint32_t slot = atomicAdd(&s_n, 1u);
-            s_rect[slot] = make_int4(s.x0, s.y0, s.x1, s.y1);
-            s_depth[slot] = s.depth;
-        }
+    synth_depth_tile(seed, W, H, n_rects, p11, znear, depth, blockIdx.x, blockIdx.y, s_rect, s_depth, &s_n);
+}
+
+// Many depth buffers in one launch (workgroup b rasterises tile b - wg_start of the job whose range holds b).
+__global__ __launch_bounds__(256) void synth_depth_batch_kernel(const SynthJob* __restrict__ jobs,
+                                                                const uint32_t* __restrict__ starts, uint32_t n_jobs)
+{
+    __shared__ int4 s_rect[kRectChunk];
+    __shared__ float s_depth[kRectChunk];
+    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_start[kLdsJobs];
+    uint32_t j;
+    if (n_jobs <= kLdsJobs) {
+        for (uint32_t k = threadIdx.x; k < n_jobs; k += 256) s_start[k] = starts[k];
+        __syncthreads();
+        j = last_le(s_start, n_jobs, blockIdx.x);
+    } else {
+        j = last_le(starts, n_jobs, blockIdx.x);
     }
-    __syncthreads();
-    const uint32_t n = s_n;
-    for (uint32_t t = threadIdx.x; t < kRasterTile * kRasterTile; t += 256) {
-        const int32_t x = tx0 + (int32_t)(t % kRasterTile), y = ty0 + (int32_t)(t / kRasterTile);
-        if (x >= (int32_t)W || y >= (int32_t)H) continue;
-        float d = synth_plane_depth((uint32_t)y, H, p11, znear);
-        for (uint32_t k = 0; k < n; ++k) {
-            const int4 rc = s_rect[k];
-            if (x >= rc.x && x < rc.z && y >= rc.y && y < rc.w) d = glsl_max(d, s_depth[k]);
-        }
-        depth[(size_t)y * W + x] = d;
-    }
+    const SynthJob J = jobs[j];
+    const uint32_t local = blockIdx.x - J.wg_start;
+    if (local == 0 && J.tail_zero)
+        for (uint32_t k = threadIdx.x; k < J.n_tail; k += 256) J.tail_zero[k] = 0ull;
+    synth_depth_tile(J.seed, J.w, J.h, J.occluders, J.p11, J.znear, J.depth, local % J.tiles_x, local / J.tiles_x,
+                     s_rect, s_depth, &s_n);
 }
 
 }  // namespace
@@ -303,6 +354,14 @@ hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, floa
 {
     hipLaunchKernelGGL(pyramid_level_kernel, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(256), 0, st, src, sw, sh, dst,
                        dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_synth_depth_batch(const SynthJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
+                                    uint32_t total_wgs, hipStream_t st)
+{
+    if (n_jobs == 0 || total_wgs == 0) return hipSuccess;
+    hipLaunchKernelGGL(synth_depth_batch_kernel, dim3(total_wgs), dim3(256), 0, st, jobs_dev, starts_dev, n_jobs);
     return hipGetLastError();
 }
 

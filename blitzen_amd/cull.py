@@ -257,6 +257,66 @@ class Context:
     def graph_destroy(self, graph: int):
         self.check(lib().bz_graph_destroy(self.handle, graph), "bz_graph_destroy")
 
+    # ------------------------------------------------------------------ serving
+    def server_start(self, budget_bytes: int, inflight: int = 4, max_batch: int = 32):
+        cfg = A.ServerCfg(hbm_budget_bytes=budget_bytes, max_inflight_batches=inflight, max_batch_requests=max_batch)
+        self.check(lib().bz_server_start(self.handle, C.byref(cfg)), "bz_server_start")
+
+    def server_stop(self):
+        self.check(lib().bz_server_stop(self.handle), "bz_server_stop")
+
+    @staticmethod
+    def request(scene: int, view: CameraViewData, depth_seed: int, depth_w: int, depth_h: int, occluders: int,
+                pass_: int = A.PASS_TEMPORAL, deadline_ns: int = 0, user_tag: int = 0) -> A.Request:
+        return A.Request(scene=scene, view=view, depth_seed=depth_seed, depth_w=depth_w, depth_h=depth_h,
+                         occluders=occluders, pass_=pass_, deadline_ns=deadline_ns, user_tag=user_tag)
+
+    def submit(self, req: A.Request) -> int:
+        t = C.c_uint64()
+        self.check(lib().bz_server_submit(self.handle, C.byref(req), C.byref(t)), "bz_server_submit")
+        return t.value
+
+    def submit_schedule(self, reqs: list[A.Request], offsets_ns) -> tuple[np.ndarray, np.ndarray]:
+        n = len(reqs)
+        arr = (A.Request * n)(*reqs)
+        offs = np.ascontiguousarray(offsets_ns, dtype=np.uint64)
+        tickets = np.zeros(n, np.uint64)
+        st = np.zeros(n, np.int32)
+        self.check(lib().bz_server_submit_schedule(self.handle, arr, offs.ctypes.data_as(C.POINTER(C.c_uint64)), n,
+                                                   tickets.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                   st.ctypes.data_as(C.POINTER(C.c_int32))), "bz_server_submit_schedule")
+        return tickets, st
+
+    def poll(self, ticket: int):
+        r = A.Result()
+        s = lib().bz_server_poll(self.handle, ticket, C.byref(r))
+        if s == A.E_NOT_READY:
+            return None
+        self.check(s, "bz_server_poll")
+        return r
+
+    def wait(self, ticket: int, timeout_s: float = 0.0) -> A.Result:
+        r = A.Result()
+        self.check(lib().bz_server_wait(self.handle, ticket, int(timeout_s * 1e9), C.byref(r)), "bz_server_wait")
+        return r
+
+    def fetch(self, ticket: int, capacity: int):
+        out = np.zeros(max(capacity, 1), INDIRECT_DRAW)
+        n = C.c_uint32()
+        self.check(lib().bz_server_fetch(self.handle, ticket, A.ptr(out), capacity, C.byref(n)), "bz_server_fetch")
+        return out[:min(n.value, capacity)], n.value
+
+    def release(self, ticket: int):
+        self.check(lib().bz_server_release(self.handle, ticket), "bz_server_release")
+
+    def cancel(self, ticket: int):
+        self.check(lib().bz_server_cancel(self.handle, ticket), "bz_server_cancel")
+
+    def server_stats(self) -> A.ServerStats:
+        st = A.ServerStats()
+        self.check(lib().bz_server_get_stats(self.handle, C.byref(st)), "bz_server_get_stats")
+        return st
+
     def synth_depth_device(self, seed: int, w: int, h: int, occluders: int, p11: float, znear: float, depth_dev: int,
                            stream=None):
         self.check(lib().bz_synth_depth_device(self.handle, seed, w, h, occluders, p11, znear, depth_dev, stream),

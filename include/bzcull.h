@@ -362,6 +362,11 @@ typedef struct bz_result {
 BZ_API bz_status bz_server_start(bz_ctx* ctx, const bz_server_cfg* cfg);
 BZ_API bz_status bz_server_stop(bz_ctx* ctx);
 BZ_API bz_status bz_server_submit(bz_ctx* ctx, const bz_request* req, uint64_t* ticket);
+/* Submits reqs[i] at offsets_ns[i] after the call starts (ascending offsets; a native busy-wait paces the
+ * arrivals, e.g. a seeded Poisson process). tickets[i] = 0 and statuses[i] != BZ_OK for a rejected request;
+ * statuses may be NULL. Returns when every request has been submitted. */
+BZ_API bz_status bz_server_submit_schedule(bz_ctx* ctx, const bz_request* reqs, const uint64_t* offsets_ns, uint32_t n,
+                                           uint64_t* tickets, bz_status* statuses);
 /* Non-blocking. BZ_E_NOT_READY while the request is pending. */
 BZ_API bz_status bz_server_poll(bz_ctx* ctx, uint64_t ticket, bz_result* out);
 /* Blocks until done or timeout_ns elapses (BZ_E_TIMEOUT). */
