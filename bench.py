@@ -146,6 +146,65 @@ def bench_batched(ctx, stream, steps: int, warmup: int):
     return out
 
 
+def bench_served(ctx, rates, duration_s: float):
+    """Poisson request streams against the serving loop over 8 resident scenes; latency = completion - submission."""
+    import numpy as np
+
+    from blitzen_amd import _capi as A
+    from blitzen_amd import workloads as W
+
+    t0 = time.perf_counter()
+    scenes = W.served_scenes(ctx)
+    log(f"bench: served scenes ({sum(n for _, n in scenes)} objects) resident in {time.perf_counter() - t0:.1f}s")
+    budget = 64 << 30
+    ctx.server_start(budget, inflight=4, max_batch=64, flags=A.SERVER_RELEASE_ON_COMPLETE)
+    out = {"workload": f"8 resident scenes {W.SERVE_SCENES} (config-1 distributions); Poisson arrivals; each request: "
+                       f"seeded camera, depth {W.SERVE_DEPTH_SIZES[0]}..{W.SERVE_DEPTH_SIZES[-1]} rasterised on the "
+                       f"device from its seed ({W.SERVE_OCCLUDERS} occluders), own pyramid, Hi-Z cull of the whole "
+                       f"scene; 64 GB HBM budget; 4 batches in flight, <= 64 requests per batch",
+           "rates": []}
+    try:
+        warm, offs, _ = W.served_requests(scenes, 2000, 0.02, seed=1)
+        tk, _ = ctx.submit_schedule(warm, offs)
+        for t in tk:
+            ctx.wait(int(t), 60)
+            ctx.release(int(t))
+        for rate in rates:
+            reqs, offs, objs = W.served_requests(scenes, rate, duration_s)
+            tickets, st = ctx.submit_schedule(reqs, offs)
+            lat, sub, comp, done_objs, bsz, bad = [], [], [], 0, [], 0
+            for t, n_obj in zip(tickets, objs):
+                if not t:
+                    bad += 1
+                    continue
+                r = ctx.wait(int(t), 120)
+                if r.status == A.OK:
+                    lat.append((r.complete_ns - r.submit_ns) / 1e6)
+                    sub.append(r.submit_ns)
+                    comp.append(r.complete_ns)
+                    done_objs += int(n_obj)
+                    bsz.append(r.batch_size)
+                else:
+                    bad += 1
+                ctx.release(int(t))
+            span = (max(comp) - min(sub)) / 1e9 if comp else float("nan")
+            out["rates"].append({
+                "offered_req_per_s": rate, "requests": len(reqs), "failed": bad,
+                "achieved_req_per_s": round(len(lat) / span, 1),
+                "object_culls_per_s": round(done_objs / span, 1),
+                "latency_ms_p50": round(pct(lat, 50), 4), "latency_ms_p99": round(pct(lat, 99), 4),
+                "latency_ms_max": round(max(lat), 4), "mean_batch": round(statistics.mean(bsz), 2)})
+            log(f"bench: served {rate}/s -> p50 {out['rates'][-1]['latency_ms_p50']} ms, "
+                f"p99 {out['rates'][-1]['latency_ms_p99']} ms, {out['rates'][-1]['achieved_req_per_s']} req/s")
+        st = ctx.server_stats()
+        out["budget_peak_bytes"] = int(st.budget_peak)
+    finally:
+        ctx.server_stop()
+        for h, _ in scenes:
+            ctx.scene_destroy(h)
+    return out
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_late_cull_c1.json"
     if p.exists():
@@ -165,6 +224,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-seconds", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -293,6 +354,9 @@ def main():
     if not args.no_batched:
         log("bench: batched views ...")
         result["batched"] = bench_batched(ctx, stream, max(5, min(args.steps, 20)), 2)
+    if not args.no_served:
+        log("bench: served requests ...")
+        result["served"] = bench_served(ctx, (2000, 8000, 32000), args.served_seconds)
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline((sc.objects, sc.transforms, sc.surfaces, sc.lods), bytes(memoryview(v)),

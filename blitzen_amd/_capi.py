@@ -111,6 +111,7 @@ OK, E_INVALID, E_OOM, E_HIP, E_CAPACITY, E_BUDGET, E_TIMEOUT, E_NOT_READY, E_NOT
 PASS_EARLY, PASS_LATE, PASS_PLAIN, PASS_TEMPORAL, PASS_ONPC = 0, 1, 2, 3, 4
 CULL_EARLY_HIZ = 0x1
 CLUSTER_FRUSTUM, CLUSTER_HIZ, CLUSTER_CONE = 0x1, 0x2, 0x4
+SERVER_RELEASE_ON_COMPLETE = 0x1
 
 _VP = C.c_void_p
 _U32, _U64, _F32, _I32 = C.c_uint32, C.c_uint64, C.c_float, C.c_int32

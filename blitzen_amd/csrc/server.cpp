@@ -40,6 +40,9 @@ uint64_t now_ns()
 
 constexpr uint64_t kAlign = 256;
 constexpr uint64_t kNone = Arena::kFail;
+// Objects per batch: each slot's compaction scratch is allocated once for this many (plus one tile per request),
+// so no allocation (and no device-wide synchronisation) happens while serving. A larger request runs alone.
+constexpr uint64_t kBatchObjects = 64ull << 20;
 
 enum ReqState : int { RQ_QUEUED, RQ_INFLIGHT, RQ_DONE };
 
@@ -159,6 +162,7 @@ void Server::release_locked(Req& q)
 bool Server::form_batch(std::vector<ReqP>& out)
 {
     const uint64_t t = now_ns();
+    uint64_t batch_objects = 0;
     while (!queue.empty() && out.size() < cfg.max_batch_requests) {
         ReqP q = queue.front();
         if (q->cancel || (q->r.deadline_ns && t > q->r.deadline_ns)) {
@@ -171,6 +175,7 @@ bool Server::form_batch(std::vector<ReqP>& out)
             cv_done.notify_all();
             continue;
         }
+        if (!out.empty() && batch_objects + q->n_objects > kBatchObjects) break;
         const uint64_t w = arena.alloc(q->work_bytes);
         if (w == kNone) break;
         const uint64_t o = arena.alloc(q->out_bytes);
@@ -183,6 +188,7 @@ bool Server::form_batch(std::vector<ReqP>& out)
         q->out_off = o;
         q->res.admit_ns = t;
         q->state = RQ_INFLIGHT;
+        batch_objects += q->n_objects;
         out.push_back(q);
     }
     stats.budget_in_use = arena.used();
@@ -257,7 +263,8 @@ bz_status Server::launch(Slot& s, std::vector<ReqP>& batch)
         cj.push_back(J);
     }
     hipError_t e = hipSuccess;
-    // compaction scratch of this slot (the slot is idle: growing it needs no wait)
+    // compaction scratch of this slot: allocated at start for kBatchObjects; only a single request larger than
+    // that (or a larger tile after a tuning change) grows it
     if (s.cs.capacity_tiles < tiles || s.cs.capacity_entries < (uint64_t)tiles * tile) {
         if (s.cs.counts) (void)hipFree(s.cs.counts);
         if (s.cs.staging) (void)hipFree(s.cs.staging);
@@ -364,7 +371,7 @@ void Server::finish(Slot& s, bz_status st)
             arena.release(q.work_off, q.work_bytes);
             q.work_off = kNone;
         }
-        if (q.cancel || st) {  // nobody will fetch: return the command region too
+        if (q.cancel || st || (cfg.flags & BZ_SERVER_RELEASE_ON_COMPLETE)) {  // no fetch follows
             if (q.out_off != kNone) {
                 arena.release(q.out_off, q.out_bytes);
                 q.out_off = kNone;
@@ -408,9 +415,10 @@ void Server::completer_loop()
             cv_done.wait(lk);
             continue;
         }
-        // batches in flight: poll again shortly (a short sleep keeps a core from spinning flat out)
+        // batches in flight: poll again at once (a sleep would add the timer's granularity, tens of us, to every
+        // request's latency)
         lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(2));
+        std::this_thread::yield();
         lk.lock();
     }
 }
@@ -477,6 +485,7 @@ bz_status bz_server_start(bz_ctx* ctx, const bz_server_cfg* cfg)
     if (!ctx || !cfg) return BZ_E_INVALID;
     if (ctx->server) return ctx->fail(BZ_E_STATE, "the server is already running");
     if (cfg->hbm_budget_bytes < (1ull << 20)) return ctx->fail(BZ_E_INVALID, "budget below 1 MiB");
+    if (cfg->flags & ~BZ_SERVER_RELEASE_ON_COMPLETE) return ctx->fail(BZ_E_INVALID, "unknown server flags");
     if (cfg->max_inflight_batches == 0 || cfg->max_inflight_batches > 16)
         return ctx->fail(BZ_E_INVALID, "max_inflight_batches must be 1..16");
     if (cfg->max_batch_requests == 0 || cfg->max_batch_requests > 1024)
@@ -499,6 +508,12 @@ bz_status bz_server_start(bz_ctx* ctx, const bz_server_cfg* cfg)
         if (e == hipSuccess) e = hipHostMalloc((void**)&s.counts_host, words * 4, hipHostMallocDefault);
         if (e == hipSuccess) std::memset(s.counts_host, 0, words * 4);
         s.err = s.counts_dev + cfg->max_batch_requests;
+        const uint64_t tiles = kBatchObjects / 256 + cfg->max_batch_requests;  // smallest tile: 256 objects
+        const uint64_t entries = kBatchObjects + (uint64_t)cfg->max_batch_requests * 2048;
+        if (e == hipSuccess) e = hipMalloc((void**)&s.cs.counts, (size_t)tiles * 4);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.cs.staging, (size_t)entries * sizeof(uint2));
+        s.cs.capacity_tiles = (uint32_t)tiles;
+        s.cs.capacity_entries = entries;
         if (e != hipSuccess) return ctx->hip_fail(e, "server slot");
     }
     e = hipStreamCreateWithFlags(&sv->fetch_st, hipStreamNonBlocking);
@@ -616,6 +631,7 @@ bz_status bz_server_fetch(bz_ctx* ctx, uint64_t ticket, bz_IndirectDrawData* cmd
     }
     bz_status st = q->res.status;
     const uint32_t n = std::min(q->res.count, capacity);
+    if (!st && n && off == kNone) st = ctx->fail(BZ_E_STATE, "the commands were released at completion");
     if (!st && n && off != kNone) {
         hipError_t e = hipMemcpyAsync(cmds, s->base + off, (size_t)n * sizeof(bz_IndirectDrawData),
                                       hipMemcpyDeviceToHost, s->fetch_st);

@@ -258,8 +258,9 @@ class Context:
         self.check(lib().bz_graph_destroy(self.handle, graph), "bz_graph_destroy")
 
     # ------------------------------------------------------------------ serving
-    def server_start(self, budget_bytes: int, inflight: int = 4, max_batch: int = 32):
-        cfg = A.ServerCfg(hbm_budget_bytes=budget_bytes, max_inflight_batches=inflight, max_batch_requests=max_batch)
+    def server_start(self, budget_bytes: int, inflight: int = 4, max_batch: int = 32, flags: int = 0):
+        cfg = A.ServerCfg(hbm_budget_bytes=budget_bytes, max_inflight_batches=inflight, max_batch_requests=max_batch,
+                          flags=flags)
         self.check(lib().bz_server_start(self.handle, C.byref(cfg)), "bz_server_start")
 
     def server_stop(self):

@@ -151,3 +151,36 @@ def algorithmic_bytes_batched(plan: list[ViewPlan], commands: int) -> int:
         texels = sum(max(1, bw >> L) * max(1, bh >> L) for L in range(mips))
         total += p.count * 48 + p.w * p.h * 4 + texels * 4 + 4
     return total + commands * 24
+
+
+SERVE_DEPTH_SIZES = [(512, 512), (800, 600), (1024, 768), (1280, 720), (1600, 900), (1920, 1080)]
+SERVE_OCCLUDERS = 128
+
+
+def served_scenes(ctx: bz.Context, seed: int = SERVE_SEED, sizes=SERVE_SCENES):
+    """The resident scenes of the serving workload (config-1 distributions). Returns [(handle, n_objects)]."""
+    out = []
+    for k, n in enumerate(sizes):
+        sc = bz.synth_scene(seed + k, n, 256, 8)
+        out.append((ctx.scene_create(sc.objects, sc.transforms, sc.surfaces, sc.lods), n))
+        del sc
+    return out
+
+
+def served_requests(scenes, rate: float, duration_s: float, seed: int = SERVE_SEED):
+    """Poisson arrivals at `rate` req/s for `duration_s`: each request picks a resident scene, a seeded camera
+    (fov U[50, 90] deg) and a depth buffer of 512x512 .. 1920x1080 with 128 occluders (late-pass Hi-Z against its
+    own pyramid). Returns (requests, arrival offsets in ns, objects per request)."""
+    rng = np.random.default_rng([seed, int(rate)])
+    n = max(1, int(rate * duration_s))
+    offsets = np.cumsum(rng.exponential(1e9 / rate, n)).astype(np.uint64)
+    reqs, objs = [], []
+    for i in range(n):
+        h, n_obj = scenes[int(rng.integers(len(scenes)))]
+        w, hh = SERVE_DEPTH_SIZES[int(rng.integers(len(SERVE_DEPTH_SIZES)))]
+        v = bz.camera_setup(float(rng.uniform(50, 90)), w, hh, 0.1, 650.0, tuple(rng.uniform(-50, 50, 3)),
+                            float(rng.uniform(-math.pi, math.pi)), float(rng.uniform(-0.5, 0.5)))
+        reqs.append(bz.Context.request(h, v, depth_seed=seed * 1000003 + int(rate) * 7 + i, depth_w=w, depth_h=hh,
+                                       occluders=SERVE_OCCLUDERS, pass_=A.PASS_TEMPORAL, user_tag=i))
+        objs.append(n_obj)
+    return reqs, offsets, np.array(objs, np.int64)

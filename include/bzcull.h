@@ -327,11 +327,12 @@ BZ_API bz_status bz_camera_setup(float fov_radians, float window_w, float window
 /* ---------------------------------------------------------------------------------------------
  * Serving: a request loop over resident scenes with continuous batching and HBM-budget admission.
  * ------------------------------------------------------------------------------------------- */
+#define BZ_SERVER_RELEASE_ON_COMPLETE 0x1u /* free each request's commands at completion (count and times kept) */
 typedef struct bz_server_cfg {
     uint64_t hbm_budget_bytes;     /* budget for per-request depth, pyramids and outputs */
     uint32_t max_inflight_batches; /* device batches in flight at once (streams) */
     uint32_t max_batch_requests;   /* upper bound of requests in one batch */
-    uint32_t flags;                /* reserved, 0 */
+    uint32_t flags;                /* BZ_SERVER_* */
     uint32_t reserved0;
     uint64_t reserved[4];
 } bz_server_cfg;
