@@ -184,3 +184,61 @@ def served_requests(scenes, rate: float, duration_s: float, seed: int = SERVE_SE
                                        occluders=SERVE_OCCLUDERS, pass_=A.PASS_TEMPORAL, user_tag=i))
         objs.append(n_obj)
     return reqs, offsets, np.array(objs, np.int64)
+
+
+CLUSTER_SEED = 0xB1172E04
+
+
+@dataclass
+class ClusterFrames:
+    """Cluster-mode workload: 1 048 576 objects over 128 surfaces with 16-128 clusters of 64 triangles each (cone
+    axis uniform on the sphere, cutoff U[-0.5, 0.9]); per frame 1% of the transforms get seeded rotation updates,
+    then the two-level cull (object pass, expansion, cluster frustum + Hi-Z + cone tests) runs against a 1920x1080
+    depth buffer's 1024x512 11-mip pyramid."""
+    ctx: bz.Context
+    scene: int
+    n_objects: int
+    transforms: np.ndarray
+    view: object
+    pyramid: int
+    buffers: list
+
+    @classmethod
+    def build(cls, ctx: bz.Context, n_objects: int = 1048576, seed: int = CLUSTER_SEED, stream=None):
+        sc = bz.synth_scene(seed, n_objects, 128, 8, clusters=(16, 128))
+        h = ctx.scene_create(sc.objects, sc.transforms, sc.surfaces, sc.lods)
+        ctx.scene_set_clusters(h, sc.clusters)
+        v = bz.camera_setup(70.0, 1920, 1080, 0.1, 650.0)
+        v.pyramidWidth, v.pyramidHeight = 1024.0, 512.0
+        depth = ctx.alloc(1920 * 1080 * 4)
+        ctx.synth_depth_device(seed, 1920, 1080, 512, v.proj5, 0.1, depth.ptr, stream)
+        pyr = ctx.pyramid_create(1024, 512, 11)
+        ctx.pyramid_build(pyr, depth.ptr, 1920, 1080, stream)
+        max_records = n_objects * int(sc.lods["clusterCount"].max())
+        disp = ctx.alloc(min(max_records, 1 << 27) * 12)
+        cmds = ctx.alloc(min(max_records, 1 << 27) * 24)
+        counts = ctx.alloc(16)
+        self = cls(ctx=ctx, scene=h, n_objects=n_objects, transforms=sc.transforms.copy(), view=v, pyramid=pyr,
+                   buffers=[depth, disp, cmds, counts])
+        self.disp, self.cmds, self.counts = disp, cmds, counts
+        return self
+
+    def updates(self, frame: int):
+        """The frame's seeded rotation updates (1% of the transforms): (ids, new transforms)."""
+        return bz.synth_transform_updates(CLUSTER_SEED, frame, self.transforms, self.n_objects // 100)
+
+    def frame(self, ids, new, stream=None):
+        self.ctx.update_transforms(self.scene, ids, new, stream)
+        cnt = self.counts
+        self.ctx.cull_clusters(self.scene, self.view, A.CLUSTER_FRUSTUM | A.CLUSTER_HIZ | A.CLUSTER_CONE, self.disp,
+                               _Ptr(cnt.ptr), self.cmds, _Ptr(cnt.ptr + 4), pyramid=self.pyramid, stream=stream)
+
+    def result_counts(self, stream=None):
+        c = self.counts.download(np.uint32, 2, stream)
+        return int(c[0]), int(c[1])
+
+    def free(self):
+        self.ctx.pyramid_destroy(self.pyramid)
+        for b in self.buffers:
+            b.free()
+        self.ctx.scene_destroy(self.scene)
