@@ -316,11 +316,12 @@ def main():
         ctx.pyramid_build(pyr, dbuf.ptr, DEPTH_W, DEPTH_H, stream)
         if ev:
             ev[1].record(stream)
-        ctx.cull(h, v, A.PASS_EARLY, cmds_e, ce, capacity=N_OBJECTS, vis_in=vin.ptr, stream=stream)
+        ctx.cull(h, v, A.PASS_EARLY, cmds_e, ce, capacity=N_OBJECTS, flags=A.CULL_FRAME_CACHE, vis_in=vin.ptr,
+                 stream=stream)
         if ev:
             ev[2].record(stream)
-        ctx.cull(h, v, A.PASS_LATE, cmds_l, cl, capacity=N_OBJECTS, pyramid=pyr, vis_in=vin.ptr, vis_out=vout.ptr,
-                 stream=stream)
+        ctx.cull(h, v, A.PASS_LATE, cmds_l, cl, capacity=N_OBJECTS, pyramid=pyr, flags=A.CULL_FRAME_CACHE,
+                 vis_in=vin.ptr, vis_out=vout.ptr, stream=stream)
         if ev:
             ev[3].record(stream)
 

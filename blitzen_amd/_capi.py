@@ -110,6 +110,7 @@ OK, E_INVALID, E_OOM, E_HIP, E_CAPACITY, E_BUDGET, E_TIMEOUT, E_NOT_READY, E_NOT
     E_INTERNAL, E_CANCELLED = 0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11, -12
 PASS_EARLY, PASS_LATE, PASS_PLAIN, PASS_TEMPORAL, PASS_ONPC = 0, 1, 2, 3, 4
 CULL_EARLY_HIZ = 0x1
+CULL_FRAME_CACHE = 0x2
 CLUSTER_FRUSTUM, CLUSTER_HIZ, CLUSTER_CONE = 0x1, 0x2, 0x4
 SERVER_RELEASE_ON_COMPLETE = 0x1
 

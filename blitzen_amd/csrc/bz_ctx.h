@@ -33,6 +33,19 @@ struct Scene {
     uint32_t* cl_cone = nullptr;
     uint2* cl_draw = nullptr;
     uint64_t bytes = 0;
+    uint64_t xf_version = 0;            // bumped by every transform update (invalidates the frame cache)
+    // frame cache: the frustum survivors of the last EARLY pass that recorded one (BZ_CULL_FRAME_CACHE)
+    struct FrameCache {
+        float4* sph = nullptr;
+        uint4* meta = nullptr;
+        uint32_t* counts = nullptr;
+        uint64_t cap_entries = 0;
+        uint32_t cap_tiles = 0;
+        bool valid = false;
+        float view_key[18] = {};        // view matrix rows and frustum constants the survivors were computed with
+        uint32_t begin = 0, n = 0, block = 0, items = 0;
+        uint64_t xf_version = 0;
+    } cache;
     // transform-update staging
     void* upd_host = nullptr;
     void* upd_dev = nullptr;

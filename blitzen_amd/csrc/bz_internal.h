@@ -67,8 +67,21 @@ constexpr int kMaxMips = 16;
 // Launchers (cull.hip). Each returns hipSuccess or the launch error.
 // The decide kernel is persistent: n_cu * per_cu workgroups (per_cu == 0: as many as are resident per CU),
 // clamped to the tile count.
+// Frame cache of a scene: the frustum survivors an EARLY pass found (view-space sphere, objectId, surfaceId, scale),
+// per tile in ascending objectId order, for the LATE pass of the same view and transforms to reuse.
+struct FrameCacheDev {
+    float4* sph;       // [tiles * tile] cx, cy, cz, r
+    uint4* meta;       // [tiles * tile] objectId, surfaceId, scale bits, 0
+    uint32_t* counts;  // [tiles] survivors per tile
+};
+
+// cache (EARLY only, may be null): also test every object of the range against the frustum and record the
+// survivors there.
 hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
-                       uint32_t per_cu, hipStream_t st);
+                       uint32_t per_cu, hipStream_t st, const FrameCacheDev* cache = nullptr);
+// LATE pass from a frame cache recorded with the same tile size (block x items) over the same range.
+hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
+                              uint32_t items, hipStream_t st);
 // starts_dev: the jobs' tile_start values as a compact ascending array (the batched kernels stage it in LDS)
 hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs, uint32_t total_tiles,
                              const CompactScratch& s, uint32_t block, uint32_t items, hipStream_t st);

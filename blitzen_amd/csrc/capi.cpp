@@ -67,9 +67,11 @@ Pyramid* Ctx::pyramid(bz_pyramid h)
 Scene::~Scene()
 {
     void* ps[] = { obj_surf, obj_xf, xf_ps, xf_q, surf_sphere, surf_lod, lod_error, lod_draw, lod_cluster, vis,
-                   cl_sphere, cl_cone, cl_draw };
+                   cl_sphere, cl_cone, cl_draw, cache.sph, cache.meta, cache.counts, upd_dev };
     for (void* p : ps)
         if (p) (void)hipFree(p);
+    if (upd_host) (void)hipHostFree(upd_host);
+    if (upd_event) (void)hipEventDestroy(upd_event);
 }
 
 Pyramid::~Pyramid()
@@ -246,6 +248,7 @@ bz_status Ctx::make_job(uint32_t pass, uint32_t flags, bz_scene sh, bz_pyramid p
     Scene* s = scene(sh);
     if (!s) return fail(BZ_E_INVALID, "unknown scene handle");
     if (pass > PASS_ONPC) return fail(BZ_E_INVALID, "unknown pass");
+    if (flags & ~(BZ_CULL_EARLY_HIZ | BZ_CULL_FRAME_CACHE)) return fail(BZ_E_INVALID, "unknown cull flags");
     if (!count_dev) return fail(BZ_E_INVALID, "count_dev is NULL");
     if (capacity && !cmds) return fail(BZ_E_INVALID, "cmds_dev is NULL with capacity > 0");
     if ((uintptr_t)cmds % 8u) return fail(BZ_E_INVALID, "cmds_dev must be 8-byte aligned");
@@ -521,6 +524,7 @@ bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const uint32_t
     if (!ids || !xf) return ctx->fail(BZ_E_INVALID, "NULL ids / transforms");
     for (uint32_t k = 0; k < n; ++k)
         if (ids[k] >= s->n_transforms) return ctx->fail(BZ_E_INVALID, "transform id out of range");
+    ++s->xf_version;  // frame-cache survivors recorded before this update no longer apply
     hipStream_t st = (hipStream_t)stream;
     // staging: one pinned block [ids | ps | q], reused once the previous upload on this scene completed
     const size_t need = (size_t)n * (4 + 32);
@@ -783,8 +787,66 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
         if (e != hipSuccess) return ctx->hip_fail(e, "count fill");
         return BZ_OK;
     }
+    Scene* scn = ctx->scene(a->scene);
+    const bool cache = (a->flags & BZ_CULL_FRAME_CACHE) != 0;
+    float key[18];
+    std::memcpy(key, job.view.m, sizeof job.view.m);
+    key[12] = job.view.fR;
+    key[13] = job.view.fL;
+    key[14] = job.view.fT;
+    key[15] = job.view.fB;
+    key[16] = job.view.znear;
+    key[17] = job.view.zfar;
+    Scene::FrameCache& fc = scn->cache;
+    if (cache && job.pass == PASS_LATE && fc.valid && fc.begin == job.begin && fc.n == job.n &&
+        fc.xf_version == scn->xf_version && std::memcmp(fc.view_key, key, sizeof key) == 0) {
+        // the EARLY pass of this view recorded the frustum survivors: read those instead of every transform
+        job.n_tiles = tiles_for(job.n, fc.block, fc.items);
+        s = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * fc.block * fc.items, &sc);
+        if (s) return s;
+        const FrameCacheDev d{ fc.sph, fc.meta, fc.counts };
+        hipError_t e = launch_late_cached(job, sc->cs, d, fc.block, fc.items, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "cached late-pass launch");
+        return BZ_OK;
+    }
     s = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * t.block * t.items, &sc);
     if (s) return s;
+    if (cache && job.pass == PASS_EARLY) {
+        const uint64_t entries = (uint64_t)job.n_tiles * t.block * t.items;
+        if (fc.cap_entries < entries || fc.cap_tiles < job.n_tiles) {
+            hipError_t e = hipStreamSynchronize(st);
+            for (void* p : { (void*)fc.sph, (void*)fc.meta, (void*)fc.counts })
+                if (p) (void)hipFree(p);
+            fc.sph = nullptr;
+            fc.meta = nullptr;
+            fc.counts = nullptr;
+            fc.cap_entries = 0;
+            fc.cap_tiles = 0;
+            if (e == hipSuccess) e = hipMalloc((void**)&fc.sph, (size_t)entries * sizeof(float4));
+            if (e == hipSuccess) e = hipMalloc((void**)&fc.meta, (size_t)entries * sizeof(uint4));
+            if (e == hipSuccess) e = hipMalloc((void**)&fc.counts, (size_t)job.n_tiles * 4);
+            if (e != hipSuccess) {
+                fc.valid = false;
+                return ctx->hip_fail(e, "frame cache allocation");
+            }
+            fc.cap_entries = entries;
+            fc.cap_tiles = job.n_tiles;
+        }
+        fc.valid = true;
+        std::memcpy(fc.view_key, key, sizeof key);
+        fc.begin = job.begin;
+        fc.n = job.n;
+        fc.block = t.block;
+        fc.items = t.items;
+        fc.xf_version = scn->xf_version;
+        const FrameCacheDev d{ fc.sph, fc.meta, fc.counts };
+        hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st, &d);
+        if (e != hipSuccess) {
+            fc.valid = false;
+            return ctx->hip_fail(e, "cull launch");
+        }
+        return BZ_OK;
+    }
     hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "cull launch");
     return BZ_OK;

@@ -47,6 +47,7 @@ struct TileSmem {
     uint32_t lvl[kMaxMips];
     uint32_t woff[ITEMS][BLOCK / 64];
     uint32_t csum[BLOCK / 64];        // cluster mode: per-wave sums of the emitting objects' cluster counts
+    uint32_t cwoff[ITEMS][BLOCK / 64];  // frame cache: survivor offsets in (item, wave) order
     uint32_t job;
 };
 
@@ -110,7 +111,8 @@ __device__ __forceinline__ void issue_vis(const CullJob& J, uint32_t tile_in_job
 
 // The surface-id and transform streams: for every object, or in EARLY only for last-frame-visible ones (which waits
 // for the visibility stream first).
-template <int ITEMS, int MODE, bool XF_ID>
+// ALL (EARLY recording a frame cache): every object's streams are loaded, not only the last-frame-visible ones.
+template <int ITEMS, int MODE, bool XF_ID, bool ALL = false>
 __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& s)
 {
     const SceneDev& S = J.scene;
@@ -122,7 +124,7 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
     }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        if (MODE != MODE_EARLY || s.active[k]) {
+        if (MODE != MODE_EARLY || ALL || s.active[k]) {
             const uint32_t x = XF_ID ? s.gi[k] : S.obj_xf[s.gi[k]];
             s.surf[k] = S.obj_surf[s.gi[k]];
             s.ps[k] = S.xf_ps[x];
@@ -131,11 +133,11 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
     }
 }
 
-template <int BLOCK, int ITEMS, int MODE, bool XF_ID>
+template <int BLOCK, int ITEMS, int MODE, bool XF_ID, bool ALL = false>
 __device__ __forceinline__ void issue_tile(const CullJob& J, uint32_t tile_in_job, Streams<ITEMS>& s)
 {
     issue_vis<BLOCK, ITEMS, MODE>(J, tile_in_job, s);
-    issue_objects<ITEMS, MODE, XF_ID>(J, s);
+    issue_objects<ITEMS, MODE, XF_ID, ALL>(J, s);
 }
 
 // Small tables into LDS (surface spheres and LOD ranges, LOD errors) plus the pyramid's level offsets. All loads of a
@@ -191,10 +193,12 @@ struct NoPrefetch {
 // `prefetch` is invoked once, after the tile's last global gather (the Hi-Z texels), so loads it issues stay in
 // flight through the rest of the tile (vmcnt retires in issue order: nothing later waits on them).
 // CL (cluster object pass): ccounts[gtile] also receives the sum of the emitting objects' LOD cluster counts.
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, bool CL = false, class Prefetch>
+// CACHE (EARLY with all streams loaded): every frustum survivor of the tile is recorded in the frame cache `fc`.
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, bool CL = false, bool CACHE = false, class Prefetch>
 __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, const Streams<ITEMS>& s,
                                              uint32_t* counts, uint2* staging, uint32_t* ccounts,
-                                             TileSmem<BLOCK, ITEMS>& sm, Prefetch&& prefetch)
+                                             TileSmem<BLOCK, ITEMS>& sm, Prefetch&& prefetch,
+                                             FrameCacheDev fc = FrameCacheDev{})
 {
     constexpr bool lds_tab = LDS_TAB;
     constexpr int NW = BLOCK / 64;
@@ -215,12 +219,17 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
 
     // ---- sphere -> view space + frustum for every object
     float cx[ITEMS], cy[ITEMS], cz[ITEMS], rad[ITEMS];
-    bool vis_new[ITEMS], emit[ITEMS];
+    bool vis_new[ITEMS], emit[ITEMS], frus[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         sphere_to_view(J.view, sph[k].x, sph[k].y, sph[k].z, sph[k].w, s.ps[k].x, s.ps[k].y, s.ps[k].z, s.ps[k].w,
                        s.q[k].x, s.q[k].y, s.q[k].z, s.q[k].w, cx[k], cy[k], cz[k], rad[k]);
-        vis_new[k] = s.active[k] && frustum_visible(J.view, cx[k], cy[k], cz[k], rad[k]);
+        if constexpr (CACHE) {
+            frus[k] = s.valid[k] && frustum_visible(J.view, cx[k], cy[k], cz[k], rad[k]);
+            vis_new[k] = frus[k] && s.active[k];
+        } else {
+            vis_new[k] = s.active[k] && frustum_visible(J.view, cx[k], cy[k], cz[k], rad[k]);
+        }
     }
     float4* cand = sm.cand + wave * 64u;
     uint2* cand2 = sm.cand2 + wave * 64u;
@@ -330,14 +339,28 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
         for (int k = 0; k < ITEMS; ++k)
             if (s.valid[k]) J.vis_out[s.gi[k]] = vis_new[k] ? 1u : 0u;
     }
-    uint32_t rank[ITEMS];
+    uint32_t rank[ITEMS], crank[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         const unsigned long long m = __ballot(emit[k]);
         rank[k] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (lane == 0) sm.woff[k][wave] = (uint32_t)__popcll(m);
+        if constexpr (CACHE) {
+            const unsigned long long mc = __ballot(frus[k]);
+            crank[k] = (uint32_t)__popcll(mc & ((1ull << lane) - 1ull));
+            if (lane == 0) sm.cwoff[k][wave] = (uint32_t)__popcll(mc);
+        }
     }
     __syncthreads();
+    if constexpr (CACHE) {
+        if (wave == (uint32_t)NW - 1u) {  // the cache's survivor offsets, scanned by another wave than the commands'
+            constexpr uint32_t E = ITEMS * NW;
+            const uint32_t c = lane < E ? (&sm.cwoff[0][0])[lane] : 0u;
+            const uint32_t incl = wave_incl_scan(c, lane);
+            if (lane < E) (&sm.cwoff[0][0])[lane] = incl - c;
+            if (lane == 63) fc.counts[gtile] = incl;
+        }
+    }
     if (wave == 0) {
         constexpr uint32_t E = ITEMS * NW;  // <= 64
         const uint32_t c = lane < E ? (&sm.woff[0][0])[lane] : 0u;
@@ -358,7 +381,17 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k)
         if (emit[k]) stage[sm.woff[k][wave] + rank[k]] = make_uint2(s.gi[k], lod_abs[k]);
-    __syncthreads();  // woff is rewritten by the next tile
+    if constexpr (CACHE) {
+        const size_t cb = (size_t)gtile * TILE;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if (frus[k]) {
+                const uint32_t pos = sm.cwoff[k][wave] + crank[k];
+                fc.sph[cb + pos] = make_float4(cx[k], cy[k], cz[k], rad[k]);
+                fc.meta[cb + pos] = make_uint4(s.gi[k], s.surf[k], __float_as_uint(s.ps[k].w), 0u);
+            }
+    }
+    __syncthreads();  // woff / cwoff are rewritten by the next tile
 }
 
 // One tile, loads and decision (non-persistent form, used by the batched kernel and the cluster object pass).
@@ -379,32 +412,32 @@ __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_j
 // Persistent decide for one view: workgroup b takes tiles b, b + G, b + 2G, ... (no workgroup waits on another, so
 // any residency is safe). Tables are staged once; the next tile's streams are issued while the current one is
 // decided (two register sets, alternating, so no register copy waits on a load in flight).
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
-__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging)
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false>
+__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging, FrameCacheDev fc)
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
     constexpr bool lds_tab = LDS_TAB;
     const uint32_t G = gridDim.x;
     uint32_t t = blockIdx.x;
     Streams<ITEMS> A, B;
-    issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, t, A);
+    issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE>(J, t, A);
     stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
     __syncthreads();
     while (true) {
         {
             const uint32_t tn = t + G;
             // past the last tile the current one is issued again (L2 hits, discarded): no branch around the loads
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, A, counts, staging, nullptr, sm, [&]() {
-                issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, tn < J.n_tiles ? tn : t, B);
-            });
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, A, counts, staging, nullptr, sm, [&]() {
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE>(J, tn < J.n_tiles ? tn : t, B);
+            }, fc);
             t = tn;
             if (t >= J.n_tiles) break;
         }
         {
             const uint32_t tn = t + G;
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, t, B, counts, staging, nullptr, sm, [&]() {
-                issue_tile<BLOCK, ITEMS, MODE, XF_ID>(J, tn < J.n_tiles ? tn : t, A);
-            });
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, B, counts, staging, nullptr, sm, [&]() {
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE>(J, tn < J.n_tiles ? tn : t, A);
+            }, fc);
             t = tn;
             if (t >= J.n_tiles) break;
         }
@@ -617,15 +650,169 @@ __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* _
     }
 }
 
+// LATE pass from a frame cache: the EARLY pass of the same view, transforms and range recorded each tile's frustum
+// survivors (view-space sphere, objectId, surfaceId, scale) in ascending objectId order. One workgroup per tile reads
+// the tile's last-frame visibility and its survivors only: objects outside the frustum are invisible (visibility
+// 0, no command), survivors take the Hi-Z test and, when newly visible, LOD selection -- the same arithmetic on the
+// same values as the full LATE pass, so the commands, count and visibility are identical.
+template <int BLOCK, int ITEMS, bool LDS_TAB>
+__global__ __launch_bounds__(BLOCK) void late_cached_kernel(CullJob J, FrameCacheDev fc, uint32_t* counts,
+                                                            uint2* staging)
+{
+    constexpr uint32_t TILE = BLOCK * ITEMS;
+    constexpr int NW = BLOCK / 64;
+    constexpr int LPT = (kLdsLods + BLOCK - 1) / BLOCK, SPT = (kLdsSurfaces + BLOCK - 1) / BLOCK;
+    __shared__ uint32_t s_prev[TILE], s_new[TILE], s_lvl[kMaxMips], s_woff[ITEMS][NW];
+    __shared__ float s_err[LDS_TAB ? kLdsLods : 1];
+    __shared__ uint2 s_lod[LDS_TAB ? kLdsSurfaces : 1];
+    const SceneDev& S = J.scene;
+    const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t base = t * TILE;  // first object of the tile, relative to the job's range
+    const uint32_t nt = J.n - base < TILE ? J.n - base : TILE;
+    // first round of loads: the tile's visibility, its survivor count and (when they fit) the LOD tables
+    const uint32_t c = fc.counts[t];
+    uint32_t v[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = (uint32_t)k * BLOCK + tid;
+        v[k] = J.vis_in[J.begin + base + (i < nt ? i : 0u)];
+    }
+    float t_err[LDS_TAB ? LPT : 1];
+    uint2 t_lod[LDS_TAB ? SPT : 1];
+    if constexpr (LDS_TAB) {
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            const uint32_t x = tid + (uint32_t)j * BLOCK;
+            t_err[j] = S.lod_error[x < S.n_lods ? x : 0u];
+        }
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const uint32_t x = tid + (uint32_t)j * BLOCK;
+            t_lod[j] = S.surf_lod[x < S.n_surfaces ? x : 0u];
+        }
+    }
+    // second round: the survivors (the count is needed for their addresses)
+    float4 sp[ITEMS];
+    uint4 mt[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t e = (uint32_t)k * BLOCK + tid;
+        const size_t slot = (size_t)base + (e < c ? e : 0u);  // the tile's cache region always exists
+        sp[k] = fc.sph[slot];
+        mt[k] = fc.meta[slot];
+    }
+    if (tid < (uint32_t)kMaxMips) {
+        uint32_t off = 0;
+        for (uint32_t L = 0; L < tid; ++L) off += level_dim(J.pyr.base_w, L) * level_dim(J.pyr.base_h, L);
+        s_lvl[tid] = off;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = (uint32_t)k * BLOCK + tid;
+        s_prev[i] = v[k];
+        s_new[i] = 0u;
+    }
+    if constexpr (LDS_TAB) {
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            const uint32_t x = tid + (uint32_t)j * BLOCK;
+            if (x < S.n_lods) s_err[x] = t_err[j];
+        }
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const uint32_t x = tid + (uint32_t)j * BLOCK;
+            if (x < S.n_surfaces) s_lod[x] = t_lod[j];
+        }
+    }
+    __syncthreads();
+    bool emit[ITEMS];
+    uint32_t lod_abs[ITEMS];
+    const uint32_t obj0 = J.begin + base;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t e = (uint32_t)k * BLOCK + tid;
+        emit[k] = false;
+        lod_abs[k] = 0u;
+        const uint32_t local = mt[k].x - obj0;
+        if (e < c && local < nt) {  // (an entry outside the tile can only come from a mismatched cache: skipped)
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+            const bool test = project_sphere(sp[k].x, sp[k].y, sp[k].z, sp[k].w, J.view.znear, J.view.p00, J.view.p11,
+                                             a0, a1, a2, a3);
+            // a sphere crossing the near plane is not tested and stays visible
+            const bool vis = test ? occlusion_visible(J.view, J.pyr, s_lvl, a0, a1, a2, a3, sp[k].z, sp[k].w) : true;
+            s_new[local] = vis ? 1u : 0u;
+            emit[k] = vis && s_prev[local] == 0u;
+            if (emit[k]) {
+                const uint2 lo = LDS_TAB ? s_lod[mt[k].y] : S.surf_lod[mt[k].y];
+                const float scale = __uint_as_float(mt[k].z);
+                const float len = sqrtf((sp[k].x * sp[k].x + sp[k].y * sp[k].y) + sp[k].z * sp[k].z);
+                const float distance = glsl_max(len - sp[k].w, 0.0f);
+                const float threshold = (distance * J.view.lodTarget) / scale;
+                uint32_t idx = 0;
+                for (uint32_t l = 1; l < lo.y; ++l)
+                    if ((LDS_TAB ? s_err[lo.x + l] : S.lod_error[lo.x + l]) < threshold) idx = l;
+                lod_abs[k] = idx + lo.x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = (uint32_t)k * BLOCK + tid;
+        if (i < nt) J.vis_out[obj0 + i] = s_new[i];
+    }
+    // commands in survivor order, i.e. ascending objectId: ranks in (item, wave) order
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const unsigned long long m = __ballot(emit[k]);
+        rank[k] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_woff[k][wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        constexpr uint32_t E = ITEMS * NW;
+        const uint32_t cc = lane < E ? (&s_woff[0][0])[lane] : 0u;
+        const uint32_t incl = wave_incl_scan(cc, lane);
+        if (lane < E) (&s_woff[0][0])[lane] = incl - cc;
+        if (lane == 63) counts[t] = incl;
+    }
+    __syncthreads();
+    uint2* stage = staging + (size_t)t * TILE;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+        if (emit[k]) stage[s_woff[k][wave] + rank[k]] = make_uint2(mt[k].x, lod_abs[k]);
+}
+
+// The cached late pass runs 256-thread workgroups (more of them resident per CU than 512-thread ones) over the
+// tile size the cache was recorded with.
+template <int ITEMS>
+hipError_t launch_late_cached_t(const CullJob& job, const CompactScratch& s, const FrameCacheDev& fc, hipStream_t st)
+{
+    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
+    if (lds)
+        hipLaunchKernelGGL((late_cached_kernel<256, ITEMS, true>), dim3(job.n_tiles), dim3(256), 0, st, job, fc,
+                           s.counts, s.staging);
+    else
+        hipLaunchKernelGGL((late_cached_kernel<256, ITEMS, false>), dim3(job.n_tiles), dim3(256), 0, st, job, fc,
+                           s.counts, s.staging);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
+                       (uint32_t)(256 * ITEMS), (const uint32_t*)s.counts, (const uint2*)s.staging);
+    return hipGetLastError();
+}
+
 // Persistent grid: `per_cu` workgroups per CU, or (per_cu == 0) as many as are resident at once per CU.
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
-void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st)
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false>
+void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st,
+                   const FrameCacheDev& fc = FrameCacheDev{})
 {
     static int resident = 0;
     if (resident == 0) {
         int r = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB>,
-                                                         BLOCK, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE>, BLOCK, 0) != hipSuccess ||
             r < 1)
             r = 1;
         resident = r;
@@ -633,30 +820,35 @@ void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t*
     uint64_t grid = (uint64_t)n_cu * (per_cu ? per_cu : (uint32_t)resident);
     if (grid > job.n_tiles) grid = job.n_tiles;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB>), dim3((uint32_t)grid), dim3(BLOCK), 0,
-                       st, job, counts, staging);
+    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE>), dim3((uint32_t)grid),
+                       dim3(BLOCK), 0, st, job, counts, staging, fc);
 }
 
-template <int BLOCK, int ITEMS, int MODE, bool HIZ>
-void launch_mode(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st)
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool CACHE = false>
+void launch_mode(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st,
+                 const FrameCacheDev& fc = FrameCacheDev{})
 {
     // identity scenes have no transform-index stream (obj_xf == nullptr): XF_ID must follow it in every variant
     const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
     const bool xf_id = job.scene.obj_xf == nullptr;
     if (xf_id) {
-        if (lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, true, true>(job, n_cu, per_cu, counts, staging, st);
-        else launch_decide<BLOCK, ITEMS, MODE, HIZ, true, false>(job, n_cu, per_cu, counts, staging, st);
+        if (lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, true, true, CACHE>(job, n_cu, per_cu, counts, staging, st, fc);
+        else launch_decide<BLOCK, ITEMS, MODE, HIZ, true, false, CACHE>(job, n_cu, per_cu, counts, staging, st, fc);
     } else {
-        if (lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, false, true>(job, n_cu, per_cu, counts, staging, st);
-        else launch_decide<BLOCK, ITEMS, MODE, HIZ, false, false>(job, n_cu, per_cu, counts, staging, st);
+        if (lds) launch_decide<BLOCK, ITEMS, MODE, HIZ, false, true, CACHE>(job, n_cu, per_cu, counts, staging, st, fc);
+        else launch_decide<BLOCK, ITEMS, MODE, HIZ, false, false, CACHE>(job, n_cu, per_cu, counts, staging, st, fc);
     }
 }
 
 template <int BLOCK, int ITEMS>
-hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n_cu, uint32_t per_cu, hipStream_t st)
+hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n_cu, uint32_t per_cu, hipStream_t st,
+                         const FrameCacheDev* fc)
 {
     const bool h = job.hiz != 0;
-    if (job.pass == PASS_EARLY) {
+    if (job.pass == PASS_EARLY && fc) {
+        if (h) launch_mode<BLOCK, ITEMS, MODE_EARLY, true, true>(job, n_cu, per_cu, s.counts, s.staging, st, *fc);
+        else launch_mode<BLOCK, ITEMS, MODE_EARLY, false, true>(job, n_cu, per_cu, s.counts, s.staging, st, *fc);
+    } else if (job.pass == PASS_EARLY) {
         if (h) launch_mode<BLOCK, ITEMS, MODE_EARLY, true>(job, n_cu, per_cu, s.counts, s.staging, st);
         else launch_mode<BLOCK, ITEMS, MODE_EARLY, false>(job, n_cu, per_cu, s.counts, s.staging, st);
     } else if (job.pass == PASS_LATE) {
@@ -692,13 +884,24 @@ hipError_t launch_cull_batch_t(const CullJob* jobs, const uint32_t* starts, uint
     X(256, 2) X(256, 4) X(512, 2) X(512, 4)
 
 hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
-                       uint32_t per_cu, hipStream_t st)
+                       uint32_t per_cu, hipStream_t st, const FrameCacheDev* cache)
 {
 #define X(B, I) \
-    if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st);
+    if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st, cache);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
+                              uint32_t items, hipStream_t st)
+{
+    switch (block * items) {
+    case 512: return launch_late_cached_t<2>(job, s, cache, st);
+    case 1024: return launch_late_cached_t<4>(job, s, cache, st);
+    case 2048: return launch_late_cached_t<8>(job, s, cache, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, uint32_t block,
