@@ -37,14 +37,15 @@ struct Scene {
     // frame cache: the frustum survivors of the last EARLY pass that recorded one (BZ_CULL_FRAME_CACHE)
     struct FrameCache {
         float4* sph = nullptr;
-        uint4* meta = nullptr;
+        uint2* meta = nullptr;
         uint32_t* counts = nullptr;
         uint64_t cap_entries = 0;
         uint32_t cap_tiles = 0;
         bool valid = false;
-        float view_key[18] = {};        // view matrix rows and frustum constants the survivors were computed with
+        float view_key[19] = {};        // view matrix rows, frustum constants and lodTarget the survivors used
         uint32_t begin = 0, n = 0, block = 0, items = 0;
         uint64_t xf_version = 0;
+        const uint32_t* vis_in = nullptr;  // the last-frame visibility the EARLY pass read (recorded per survivor)
     } cache;
     // transform-update staging
     void* upd_host = nullptr;

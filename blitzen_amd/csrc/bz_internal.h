@@ -67,11 +67,12 @@ constexpr int kMaxMips = 16;
 // Launchers (cull.hip). Each returns hipSuccess or the launch error.
 // The decide kernel is persistent: n_cu * per_cu workgroups (per_cu == 0: as many as are resident per CU),
 // clamped to the tile count.
-// Frame cache of a scene: the frustum survivors an EARLY pass found (view-space sphere, objectId, surfaceId, scale),
-// per tile in ascending objectId order, for the LATE pass of the same view and transforms to reuse.
+// Frame cache of a scene: the frustum survivors an EARLY pass found (view-space sphere, objectId, LOD index and the
+// last-frame visibility it read), per tile in ascending objectId order, for the LATE pass of the same view,
+// transforms and visibility input to reuse.
 struct FrameCacheDev {
     float4* sph;       // [tiles * tile] cx, cy, cz, r
-    uint4* meta;       // [tiles * tile] objectId, surfaceId, scale bits, 0
+    uint2* meta;       // [tiles * tile] objectId, LOD index | last-frame visible << 31
     uint32_t* counts;  // [tiles] survivors per tile
 };
 

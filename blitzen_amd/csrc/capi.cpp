@@ -583,6 +583,7 @@ bz_status bz_scene_set_visibility(bz_ctx* ctx, bz_scene scene, const uint32_t* v
     Scene* s = ctx->scene(scene);
     if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
     if (n != s->n_objects || (n && !vis)) return ctx->fail(BZ_E_INVALID, "visibility size must equal n_objects");
+    s->cache.valid = false;  // the frame cache holds the last-frame visibility its EARLY pass read
     if (!n) return BZ_OK;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(s->vis, vis, (size_t)n * 4, hipMemcpyHostToDevice, st);
@@ -597,6 +598,7 @@ bz_status bz_scene_get_visibility(bz_ctx* ctx, bz_scene scene, uint32_t* vis, ui
     Scene* s = ctx->scene(scene);
     if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
     if (n != s->n_objects || (n && !vis)) return ctx->fail(BZ_E_INVALID, "visibility size must equal n_objects");
+    s->cache.valid = false;  // the frame cache holds the last-frame visibility its EARLY pass read
     if (!n) return BZ_OK;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(vis, s->vis, (size_t)n * 4, hipMemcpyDeviceToHost, st);
@@ -789,8 +791,9 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
     }
     Scene* scn = ctx->scene(a->scene);
     const bool cache = (a->flags & BZ_CULL_FRAME_CACHE) != 0;
-    float key[18];
+    float key[19];
     std::memcpy(key, job.view.m, sizeof job.view.m);
+    key[18] = job.view.lodTarget;
     key[12] = job.view.fR;
     key[13] = job.view.fL;
     key[14] = job.view.fT;
@@ -799,7 +802,8 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
     key[17] = job.view.zfar;
     Scene::FrameCache& fc = scn->cache;
     if (cache && job.pass == PASS_LATE && fc.valid && fc.begin == job.begin && fc.n == job.n &&
-        fc.xf_version == scn->xf_version && std::memcmp(fc.view_key, key, sizeof key) == 0) {
+        fc.xf_version == scn->xf_version && fc.vis_in == job.vis_in &&
+        std::memcmp(fc.view_key, key, sizeof key) == 0) {
         // the EARLY pass of this view recorded the frustum survivors: read those instead of every transform
         job.n_tiles = tiles_for(job.n, fc.block, fc.items);
         s = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * fc.block * fc.items, &sc);
@@ -823,7 +827,7 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
             fc.cap_entries = 0;
             fc.cap_tiles = 0;
             if (e == hipSuccess) e = hipMalloc((void**)&fc.sph, (size_t)entries * sizeof(float4));
-            if (e == hipSuccess) e = hipMalloc((void**)&fc.meta, (size_t)entries * sizeof(uint4));
+            if (e == hipSuccess) e = hipMalloc((void**)&fc.meta, (size_t)entries * sizeof(uint2));
             if (e == hipSuccess) e = hipMalloc((void**)&fc.counts, (size_t)job.n_tiles * 4);
             if (e != hipSuccess) {
                 fc.valid = false;
@@ -839,6 +843,7 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
         fc.block = t.block;
         fc.items = t.items;
         fc.xf_version = scn->xf_version;
+        fc.vis_in = job.vis_in;
         const FrameCacheDev d{ fc.sph, fc.meta, fc.counts };
         hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st, &d);
         if (e != hipSuccess) {

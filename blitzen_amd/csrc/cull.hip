@@ -283,17 +283,21 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
         prefetch();
     }
 
-    // ---- LOD selection on the wave's emitting objects, compacted onto dense lanes in chunks of 64
+    // ---- LOD selection on the wave's emitting objects (with a frame cache: on every frustum survivor, a superset),
+    // compacted onto dense lanes in chunks of 64
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) emit[k] = late ? (vis_new[k] && s.vis[k] == 0u) : vis_new[k];
+    bool lsel[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) lsel[k] = CACHE ? frus[k] : emit[k];
     uint32_t eslot[ITEMS], lod_abs[ITEMS];
-    const uint32_t ne = wave_compact<ITEMS>(emit, eslot, lane);
+    const uint32_t ne = wave_compact<ITEMS>(lsel, eslot, lane);
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) lod_abs[k] = 0u;
     for (uint32_t c0 = 0; c0 < ne; c0 += 64u) {
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k)
-            if (emit[k] && eslot[k] - c0 < 64u) {
+            if (lsel[k] && eslot[k] - c0 < 64u) {
                 cand[eslot[k] - c0] = make_float4(cx[k], cy[k], cz[k], rad[k]);
                 cand2[eslot[k] - c0] = make_uint2(__float_as_uint(s.ps[k].w), s.surf[k]);
             }
@@ -319,7 +323,7 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
         wave_lds_sync();
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k)
-            if (emit[k] && eslot[k] - c0 < 64u) lod_abs[k] = cand2[eslot[k] - c0].x;
+            if (lsel[k] && eslot[k] - c0 < 64u) lod_abs[k] = cand2[eslot[k] - c0].x;
         wave_lds_sync();
     }
 
@@ -388,7 +392,7 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
             if (frus[k]) {
                 const uint32_t pos = sm.cwoff[k][wave] + crank[k];
                 fc.sph[cb + pos] = make_float4(cx[k], cy[k], cz[k], rad[k]);
-                fc.meta[cb + pos] = make_uint4(s.gi[k], s.surf[k], __float_as_uint(s.ps[k].w), 0u);
+                fc.meta[cb + pos] = make_uint2(s.gi[k], lod_abs[k] | (s.vis[k] != 0u ? 0x80000000u : 0u));
             }
     }
     __syncthreads();  // woff / cwoff are rewritten by the next tile
@@ -655,151 +659,129 @@ __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* _
 // the tile's last-frame visibility and its survivors only: objects outside the frustum are invisible (visibility
 // 0, no command), survivors take the Hi-Z test and, when newly visible, LOD selection -- the same arithmetic on the
 // same values as the full LATE pass, so the commands, count and visibility are identical.
-template <int BLOCK, int ITEMS, bool LDS_TAB>
-__global__ __launch_bounds__(BLOCK) void late_cached_kernel(CullJob J, FrameCacheDev fc, uint32_t* counts,
-                                                            uint2* staging)
+constexpr uint32_t kSuperTile = 4096;  // objects per workgroup of the cached late pass
+constexpr uint32_t kLcBlock = 512;
+
+// A workgroup takes a super-tile of kSuperTile objects (kSuperTile / TILE recorded tiles): the survivors of its
+// tiles, each with its LOD and the last-frame visibility the EARLY pass read, and the Hi-Z test; then the
+// visibility of the whole super-tile (coalesced; 0 outside the frustum) and the newly visible survivors' commands
+// in survivor order. Neither the visibility buffer nor the LOD tables are read.
+template <int TILE>
+__global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameCacheDev fc, uint32_t n_rec_tiles,
+                                                               uint32_t* counts, uint2* staging)
 {
-    constexpr uint32_t TILE = BLOCK * ITEMS;
-    constexpr int NW = BLOCK / 64;
-    constexpr int LPT = (kLdsLods + BLOCK - 1) / BLOCK, SPT = (kLdsSurfaces + BLOCK - 1) / BLOCK;
-    __shared__ uint32_t s_prev[TILE], s_new[TILE], s_lvl[kMaxMips], s_woff[ITEMS][NW];
-    __shared__ float s_err[LDS_TAB ? kLdsLods : 1];
-    __shared__ uint2 s_lod[LDS_TAB ? kLdsSurfaces : 1];
-    const SceneDev& S = J.scene;
-    const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t base = t * TILE;  // first object of the tile, relative to the job's range
-    const uint32_t nt = J.n - base < TILE ? J.n - base : TILE;
-    // first round of loads: the tile's visibility, its survivor count and (when they fit) the LOD tables
-    const uint32_t c = fc.counts[t];
-    uint32_t v[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = (uint32_t)k * BLOCK + tid;
-        v[k] = J.vis_in[J.begin + base + (i < nt ? i : 0u)];
+    constexpr uint32_t BLOCK = kLcBlock, NW = BLOCK / 64;
+    constexpr uint32_t S = kSuperTile / TILE;            // recorded tiles per super-tile
+    constexpr uint32_t OPT = kSuperTile / BLOCK;         // objects per thread
+    constexpr uint32_t U = 2;                            // survivors per thread in the first round
+    __shared__ uint32_t s_new[kSuperTile / 32];
+    __shared__ uint32_t s_lvl[kMaxMips], s_cnt[S + 1], s_woff[U][NW], s_base;
+    const uint32_t st = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t base = st * kSuperTile;  // first object of the super-tile, relative to the job's range
+    const uint32_t nt = J.n - base < kSuperTile ? J.n - base : kSuperTile;
+    const uint32_t obj0 = J.begin + base;
+    uint32_t cnt = 0;
+    if (tid < S) {
+        const uint32_t t = st * S + tid;
+        cnt = t < n_rec_tiles ? fc.counts[t] : 0u;
     }
-    float t_err[LDS_TAB ? LPT : 1];
-    uint2 t_lod[LDS_TAB ? SPT : 1];
-    if constexpr (LDS_TAB) {
+    if (wave == 0) {  // prefix of the recorded tiles' counts (S <= 8 lanes)
+        uint32_t x = lane < S ? cnt : 0u;
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) {
-            const uint32_t x = tid + (uint32_t)j * BLOCK;
-            t_err[j] = S.lod_error[x < S.n_lods ? x : 0u];
+        for (int off = 1; off < 8; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= (uint32_t)off) x += y;
         }
-#pragma unroll
-        for (int j = 0; j < SPT; ++j) {
-            const uint32_t x = tid + (uint32_t)j * BLOCK;
-            t_lod[j] = S.surf_lod[x < S.n_surfaces ? x : 0u];
-        }
-    }
-    // second round: the survivors (the count is needed for their addresses)
-    float4 sp[ITEMS];
-    uint4 mt[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t e = (uint32_t)k * BLOCK + tid;
-        const size_t slot = (size_t)base + (e < c ? e : 0u);  // the tile's cache region always exists
-        sp[k] = fc.sph[slot];
-        mt[k] = fc.meta[slot];
+        if (lane < S) s_cnt[lane + 1] = x;
+        if (lane == 0) s_cnt[0] = 0;
     }
     if (tid < (uint32_t)kMaxMips) {
         uint32_t off = 0;
         for (uint32_t L = 0; L < tid; ++L) off += level_dim(J.pyr.base_w, L) * level_dim(J.pyr.base_h, L);
         s_lvl[tid] = off;
     }
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = (uint32_t)k * BLOCK + tid;
-        s_prev[i] = v[k];
-        s_new[i] = 0u;
-    }
-    if constexpr (LDS_TAB) {
-#pragma unroll
-        for (int j = 0; j < LPT; ++j) {
-            const uint32_t x = tid + (uint32_t)j * BLOCK;
-            if (x < S.n_lods) s_err[x] = t_err[j];
-        }
-#pragma unroll
-        for (int j = 0; j < SPT; ++j) {
-            const uint32_t x = tid + (uint32_t)j * BLOCK;
-            if (x < S.n_surfaces) s_lod[x] = t_lod[j];
-        }
-    }
+    for (uint32_t x = tid; x < kSuperTile / 32; x += BLOCK) s_new[x] = 0u;
+    if (tid == 0) s_base = 0;
     __syncthreads();
-    bool emit[ITEMS];
-    uint32_t lod_abs[ITEMS];
-    const uint32_t obj0 = J.begin + base;
+    const uint32_t total = s_cnt[S];
+    uint2* stage = staging + (size_t)st * kSuperTile;
+    // survivors in rounds of BLOCK x U (normally one round); g = k * BLOCK + tid is the survivor's rank in the
+    // super-tile, i.e. ascending objectId
+    for (uint32_t g0 = 0; g0 < total; g0 += BLOCK * U) {
+        float4 sp[U];
+        uint2 mt[U];
+        bool have[U];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t e = (uint32_t)k * BLOCK + tid;
-        emit[k] = false;
-        lod_abs[k] = 0u;
-        const uint32_t local = mt[k].x - obj0;
-        if (e < c && local < nt) {  // (an entry outside the tile can only come from a mismatched cache: skipped)
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-            const bool test = project_sphere(sp[k].x, sp[k].y, sp[k].z, sp[k].w, J.view.znear, J.view.p00, J.view.p11,
-                                             a0, a1, a2, a3);
-            // a sphere crossing the near plane is not tested and stays visible
-            const bool vis = test ? occlusion_visible(J.view, J.pyr, s_lvl, a0, a1, a2, a3, sp[k].z, sp[k].w) : true;
-            s_new[local] = vis ? 1u : 0u;
-            emit[k] = vis && s_prev[local] == 0u;
-            if (emit[k]) {
-                const uint2 lo = LDS_TAB ? s_lod[mt[k].y] : S.surf_lod[mt[k].y];
-                const float scale = __uint_as_float(mt[k].z);
-                const float len = sqrtf((sp[k].x * sp[k].x + sp[k].y * sp[k].y) + sp[k].z * sp[k].z);
-                const float distance = glsl_max(len - sp[k].w, 0.0f);
-                const float threshold = (distance * J.view.lodTarget) / scale;
-                uint32_t idx = 0;
-                for (uint32_t l = 1; l < lo.y; ++l)
-                    if ((LDS_TAB ? s_err[lo.x + l] : S.lod_error[lo.x + l]) < threshold) idx = l;
-                lod_abs[k] = idx + lo.x;
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t g = g0 + k * BLOCK + tid;
+            have[k] = g < total;
+            uint32_t j = 0;  // recorded tile holding survivor g
+#pragma unroll
+            for (uint32_t q = 1; q < S; ++q) j += g >= s_cnt[q] ? 1u : 0u;
+            const uint32_t e = have[k] ? g - s_cnt[j] : 0u;
+            const size_t slot = (size_t)(st * S + (have[k] ? j : 0u)) * TILE + e;  // in the cache region
+            sp[k] = fc.sph[slot];
+            mt[k] = fc.meta[slot];
+        }
+        bool emit[U];
+        uint32_t lod_abs[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            emit[k] = false;
+            lod_abs[k] = 0u;
+            const uint32_t local = mt[k].x - obj0;
+            if (have[k] && local < nt) {  // (an entry outside the super-tile can only come from a mismatched cache)
+                float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+                const bool test = project_sphere(sp[k].x, sp[k].y, sp[k].z, sp[k].w, J.view.znear, J.view.p00,
+                                                 J.view.p11, a0, a1, a2, a3);
+                // a sphere crossing the near plane is not tested and stays visible
+                const bool vis =
+                    test ? occlusion_visible(J.view, J.pyr, s_lvl, a0, a1, a2, a3, sp[k].z, sp[k].w) : true;
+                if (vis) atomicOr(&s_new[local >> 5], 1u << (local & 31u));
+                emit[k] = vis && (mt[k].y & 0x80000000u) == 0u;
+                lod_abs[k] = mt[k].y & 0x7fffffffu;
             }
         }
-    }
-    __syncthreads();
+        // commands of this round in survivor order: ranks in (k, wave, lane) order after the earlier rounds
+        uint32_t rank[U];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = (uint32_t)k * BLOCK + tid;
-        if (i < nt) J.vis_out[obj0 + i] = s_new[i];
-    }
-    // commands in survivor order, i.e. ascending objectId: ranks in (item, wave) order
-    uint32_t rank[ITEMS];
+        for (uint32_t k = 0; k < U; ++k) {
+            const unsigned long long m = __ballot(emit[k]);
+            rank[k] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) s_woff[k][wave] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (wave == 0) {
+            constexpr uint32_t E = U * NW;
+            const uint32_t c = lane < E ? (&s_woff[0][0])[lane] : 0u;
+            const uint32_t incl = wave_incl_scan(c, lane);
+            if (lane < E) (&s_woff[0][0])[lane] = s_base + incl - c;
+            if (lane == 63) s_base += incl;
+        }
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const unsigned long long m = __ballot(emit[k]);
-        rank[k] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) s_woff[k][wave] = (uint32_t)__popcll(m);
+        for (uint32_t k = 0; k < U; ++k)
+            if (emit[k]) stage[s_woff[k][wave] + rank[k]] = make_uint2(mt[k].x, lod_abs[k]);
+        __syncthreads();  // s_woff is rewritten by the next round
     }
-    __syncthreads();
-    if (wave == 0) {
-        constexpr uint32_t E = ITEMS * NW;
-        const uint32_t cc = lane < E ? (&s_woff[0][0])[lane] : 0u;
-        const uint32_t incl = wave_incl_scan(cc, lane);
-        if (lane < E) (&s_woff[0][0])[lane] = incl - cc;
-        if (lane == 63) counts[t] = incl;
-    }
-    __syncthreads();
-    uint2* stage = staging + (size_t)t * TILE;
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k)
-        if (emit[k]) stage[s_woff[k][wave] + rank[k]] = make_uint2(mt[k].x, lod_abs[k]);
+    for (uint32_t k = 0; k < OPT; ++k) {
+        const uint32_t i = k * BLOCK + tid;
+        if (i < nt) J.vis_out[obj0 + i] = (s_new[i >> 5] >> (i & 31u)) & 1u;
+    }
+    if (tid == 0) counts[st] = s_base;
 }
 
-// The cached late pass runs 256-thread workgroups (more of them resident per CU than 512-thread ones) over the
-// tile size the cache was recorded with.
-template <int ITEMS>
-hipError_t launch_late_cached_t(const CullJob& job, const CompactScratch& s, const FrameCacheDev& fc, hipStream_t st)
+template <int TILE>
+hipError_t launch_late_cached_t(const CullJob& job, const CompactScratch& s, const FrameCacheDev& fc,
+                                uint32_t n_rec_tiles, hipStream_t st)
 {
-    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
-    if (lds)
-        hipLaunchKernelGGL((late_cached_kernel<256, ITEMS, true>), dim3(job.n_tiles), dim3(256), 0, st, job, fc,
-                           s.counts, s.staging);
-    else
-        hipLaunchKernelGGL((late_cached_kernel<256, ITEMS, false>), dim3(job.n_tiles), dim3(256), 0, st, job, fc,
-                           s.counts, s.staging);
+    hipLaunchKernelGGL((late_cached_kernel<TILE>), dim3(job.n_tiles), dim3(kLcBlock), 0, st, job, fc, n_rec_tiles,
+                       s.counts, s.staging);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
-                       (uint32_t)(256 * ITEMS), (const uint32_t*)s.counts, (const uint2*)s.staging);
+                       kSuperTile, (const uint32_t*)s.counts, (const uint2*)s.staging);
     return hipGetLastError();
 }
 
@@ -893,13 +875,18 @@ hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t blo
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
+hipError_t launch_late_cached(const CullJob& job_in, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
                               uint32_t items, hipStream_t st)
 {
-    switch (block * items) {
-    case 512: return launch_late_cached_t<2>(job, s, cache, st);
-    case 1024: return launch_late_cached_t<4>(job, s, cache, st);
-    case 2048: return launch_late_cached_t<8>(job, s, cache, st);
+    // the pass runs over super-tiles; the recorded tiles (block x items objects) are what the cache holds
+    CullJob job = job_in;
+    const uint32_t rec = block * items;
+    const uint32_t n_rec_tiles = (job.n + rec - 1) / rec;
+    job.n_tiles = (job.n + kSuperTile - 1) / kSuperTile;
+    switch (rec) {
+    case 512: return launch_late_cached_t<512>(job, s, cache, n_rec_tiles, st);
+    case 1024: return launch_late_cached_t<1024>(job, s, cache, n_rec_tiles, st);
+    case 2048: return launch_late_cached_t<2048>(job, s, cache, n_rec_tiles, st);
     default: return hipErrorInvalidValue;
     }
 }

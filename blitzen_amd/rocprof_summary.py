@@ -2,10 +2,12 @@
 
     python -m blitzen_amd.rocprof_summary trace <results.db> [--title T] [--command C]
         per-kernel dispatch count and average / min / max / total duration (kernel trace, SQLite output);
-    python -m blitzen_amd.rocprof_summary traffic <fetch.csv> <write.csv> --calib-mib N [--command C] [--json OUT]
-        HBM bytes per late pass (decide_kernel in LATE mode + the emit_kernel dispatched after it) from a FETCH_SIZE
-        and a WRITE_SIZE counter run (CSV output). Each counter is scaled by what it reads for the runtime's
-        device-to-device copy kernel of N MiB in the same run (FETCH_SIZE = N MiB read, WRITE_SIZE = N MiB written).
+    python -m blitzen_amd.rocprof_summary traffic <fetch.csv> <write.csv> --calib-mib N [--frame] [--command C]
+                                              [--json OUT]
+        HBM bytes per late pass (decide_kernel in LATE mode + the emit_kernel dispatched after it), or with --frame
+        per C1 frame (every dispatch from one pyramid build to the next), from a FETCH_SIZE and a WRITE_SIZE counter
+        run (CSV output). Each counter is scaled by what it reads for the runtime's device-to-device copy kernel of
+        N MiB in the same run (FETCH_SIZE = N MiB read, WRITE_SIZE = N MiB written).
 """
 from __future__ import annotations
 
@@ -31,9 +33,9 @@ def demangle_short(name: str) -> str:
     if "<" in n and "(" in n:  # already demangled: keep the template, drop the parameter list
         n = n[:n.index("(")] if n.index("(") > n.index("<") else n
         return n.replace("bz::(anonymous namespace)::", "").replace("void ", "")
-    for key in ("decide_kernel", "decide_batch_kernel", "decide_cluster_kernel", "emit_kernel", "emit_batch_kernel",
-                "pyramid_kernel", "pyramid_batch_kernel", "synth_depth_kernel", "synth_depth_batch_kernel",
-                "cluster_expand_kernel", "cluster_emit_kernel", "fill_u32_kernel", "scatter_transforms_kernel"):
+    m = re.search(r"\d+([a-z_]+_kernel)", n)  # mangled: <length><name> of the kernel
+    keys = (m.group(1),) if m else ()
+    for key in keys:
         if key in n:
             i = n.index(key) + len(key)
             args = n[i:]
@@ -94,6 +96,51 @@ def late_pass_values(counters):
     return out
 
 
+PYRAMID = re.compile(r"pyramid_kernel\(|14pyramid_kernelE")
+
+
+def frame_values(counters):
+    """Per frame: the sum over the dispatches from one single-view pyramid build up to the next (the frame's
+    pyramid, EARLY and LATE launches), skipping copies and fills."""
+    ids = sorted(counters)
+    starts = [k for k, did in enumerate(ids) if PYRAMID.search(counters[did][0])]
+    out = []
+    for a, b in zip(starts, starts[1:]):
+        tot = 0.0
+        for did in ids[a:b]:
+            name, val = counters[did]
+            if "rocclr" in name or "fill_u32" in name:
+                continue
+            tot += val
+        out.append(tot)
+    return out
+
+
+def frame_traffic(fetch_csv: str, write_csv: str, calib_mib: float, command: str):
+    fc, wc = read_counters(fetch_csv), read_counters(write_csv)
+    known_kib = calib_mib * 1024.0
+    f_copy = [v for n, v in fc.values() if COPY_KERNEL in n and v > 0.5 * known_kib * 0.25]
+    w_copy = [v for n, v in wc.values() if COPY_KERNEL in n and v > 0.5 * known_kib * 0.25]
+    if not f_copy or not w_copy:
+        raise SystemExit("calibration copy kernel not found in the counter files")
+    f_scale = known_kib / statistics.median(f_copy)
+    w_scale = known_kib / statistics.median(w_copy)
+    ff, ww = frame_values(fc), frame_values(wc)
+    if not ff or not ww:
+        raise SystemExit("no complete frame found")
+    fetch_kib, write_kib = statistics.median(ff), statistics.median(ww)
+    return {
+        "unit": "C1 frame (pyramid + EARLY + LATE launches)",
+        "bytes_per_launch": round((fetch_kib * f_scale + write_kib * w_scale) * 1024.0),
+        "fetch_kib_raw": round(fetch_kib, 1), "write_kib_raw": round(write_kib, 1),
+        "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+        "calibration": f"{COPY_KERNEL} of {calib_mib} MiB: FETCH_SIZE median {statistics.median(f_copy):.1f} KiB, "
+                       f"WRITE_SIZE median {statistics.median(w_copy):.1f} KiB",
+        "frames": min(len(ff), len(ww)),
+        "command": command,
+    }
+
+
 def traffic(fetch_csv: str, write_csv: str, calib_mib: float, command: str):
     fc, wc = read_counters(fetch_csv), read_counters(write_csv)
     known_kib = calib_mib * 1024.0
@@ -138,6 +185,7 @@ def main(argv=None):
     r.add_argument("--calib-mib", type=float, required=True)
     r.add_argument("--command", default="")
     r.add_argument("--json")
+    r.add_argument("--frame", action="store_true", help="per C1 frame instead of per late pass")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         text = trace(a.db, a.title, a.command)
@@ -145,7 +193,7 @@ def main(argv=None):
             Path(a.out).write_text(text)
         sys.stdout.write(text)
     else:
-        res = traffic(a.fetch_csv, a.write_csv, a.calib_mib, a.command)
+        res = (frame_traffic if a.frame else traffic)(a.fetch_csv, a.write_csv, a.calib_mib, a.command)
         if a.json:
             Path(a.json).write_text(json.dumps(res, indent=1) + "\n")
         print(json.dumps(res, indent=1))

@@ -215,10 +215,12 @@ BZ_API void bz_pyramid_reference_extent(uint32_t w, uint32_t h, uint32_t* base_w
 /* Flags */
 #define BZ_CULL_EARLY_HIZ 0x1u /* EARLY pass also tests against `pyramid` (the previous frame's pyramid) */
 /* Frame cache. EARLY: also runs the frustum test on every object of the range (last-frame visible or not) and keeps
- * the survivors' view-space spheres in the scene. LATE: when the scene holds survivors recorded with the same view
- * matrix, frustum, object range and tile size and no transform update since, reads only those and the visibility
- * instead of every object's transform; otherwise runs in full. Outputs are identical either way. The EARLY and
- * LATE calls go on the same stream (or the caller orders them). */
+ * the survivors' view-space spheres and last-frame visibility in the scene. LATE: when the scene holds survivors
+ * recorded with the same view matrix, frustum, object range and visibility input, and no transform update or
+ * bz_scene_set_visibility since, reads only those (not every object's transform and visibility) and writes the
+ * visibility of the range; otherwise runs in full. Outputs are identical either way. As in the reference frame,
+ * the visibility input is not modified between the two passes, and both calls go on the same stream (or the
+ * caller orders them). */
 #define BZ_CULL_FRAME_CACHE 0x2u
 
 typedef struct bz_cull_args {

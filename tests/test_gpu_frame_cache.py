@@ -134,6 +134,9 @@ def test_frame_cache_not_used_for_another_view_or_range(ctx, pyramid):
     h = ctx.scene_create(*scene)
     v = H.view(pyr=(1024, 512))
     frame(ctx, h, scene, v, pyramid, bz.synth_visibility(7, n, 0.5), late_view=H.view(pyr=(1024, 512), yaw=0.01))
+    finer = H.view(pyr=(1024, 512))
+    finer.lodTarget = v.lodTarget * 0.25  # same frustum, other LOD choices: the recorded LODs must not be used
+    frame(ctx, h, scene, v, pyramid, bz.synth_visibility(11, n, 0.5), late_view=finer)
     # another range for the late pass: the full pass runs
     p, opyr = pyramid
     vis0 = bz.synth_visibility(8, n, 0.5)
@@ -154,3 +157,35 @@ def test_frame_cache_not_used_for_another_view_or_range(ctx, pyramid):
     np.testing.assert_array_equal(got[0], exp[0])
     ctx.scene_destroy(h)
     ctx.scene_destroy(h2)
+
+
+def test_frame_cache_keyed_on_visibility_input(ctx, pyramid):
+    """A late pass reading another visibility buffer than the recording early pass, or after the scene's visibility
+    was replaced, runs in full and still matches the oracle."""
+    n = 50000
+    sc = H.c1_scene(n)
+    scene = H.as_tuple(sc)
+    h = ctx.scene_create(*scene)
+    v = H.view(pyr=(1024, 512))
+    p, opyr = pyramid
+    vis_a = bz.synth_visibility(9, n, 0.5)
+    vis_b = bz.synth_visibility(10, n, 0.5)
+    ctx.set_visibility(h, vis_a)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)  # records with the scene's buffer (vis_a)
+    other = ctx.alloc(n * 4)
+    other.upload(vis_b)
+    got = run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC, vis_in=other.ptr)  # reads vis_b in place
+    vexp = vis_b.copy()
+    exp = orc.cull(scene, H.view_bytes(v), A.PASS_LATE, opyr, vis_in=vis_b, vis_out=vexp)
+    assert got[1] == exp[1]
+    np.testing.assert_array_equal(got[0], exp[0])
+    np.testing.assert_array_equal(other.download(np.uint32, n), vexp)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)  # records vis_a again
+    ctx.set_visibility(h, vis_b)  # replaced: the recorded bits no longer apply
+    got = run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC)
+    vexp = vis_b.copy()
+    exp = orc.cull(scene, H.view_bytes(v), A.PASS_LATE, opyr, vis_in=vis_b, vis_out=vexp)
+    np.testing.assert_array_equal(got[0], exp[0])
+    np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
+    other.free()
+    ctx.scene_destroy(h)
