@@ -25,8 +25,11 @@ CLUSTER = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("coneAxis", "i1", 
 INDIRECT_DRAW = np.dtype([("objectId", "<u4"), ("indexCount", "<u4"), ("instanceCount", "<u4"), ("firstIndex", "<u4"),
                           ("vertexOffset", "<i4"), ("firstInstance", "<u4")])
 CLUSTER_DISPATCH = np.dtype([("objectId", "<u4"), ("lodIndex", "<u4"), ("clusterId", "<u4")])
+INDIRECT_DRAW32 = np.dtype([("objId", "<u4"), ("indexCount", "<u4"), ("instanceCount", "<u4"), ("firstIndex", "<u4"),
+                            ("vertexOffset", "<i4"), ("firstInstance", "<u4"), ("padding", "<u4", 2)])
+LOD_INSTANCE_COUNTER = np.dtype([("instanceOffset", "<u4"), ("instanceCount", "<u4")])
 for _dt, _sz in ((RENDER_OBJECT, 8), (MESH_TRANSFORM, 32), (PRIMITIVE_SURFACE, 32), (LOD_DATA, 32), (CLUSTER, 32),
-                 (INDIRECT_DRAW, 24), (CLUSTER_DISPATCH, 12)):
+                 (INDIRECT_DRAW, 24), (CLUSTER_DISPATCH, 12), (INDIRECT_DRAW32, 32), (LOD_INSTANCE_COUNTER, 8)):
     assert _dt.itemsize == _sz, (_dt, _sz)
 
 
@@ -57,6 +60,12 @@ class ClusterCullArgs(C.Structure):
                 ("view", C.POINTER(CameraViewData)), ("dispatch_dev", C.c_void_p), ("dispatch_count_dev", C.c_void_p),
                 ("dispatch_capacity", C.c_uint32), ("cmds_dev", C.c_void_p), ("count_dev", C.c_void_p),
                 ("capacity", C.c_uint32)]
+
+
+class InstancedCullArgs(C.Structure):
+    _fields_ = [("scene", C.c_uint64), ("view", C.POINTER(CameraViewData)), ("instances_per_lod", C.c_uint32),
+                ("capacity", C.c_uint32), ("instance_ids_dev", C.c_void_p), ("counters_dev", C.c_void_p),
+                ("cmds_dev", C.c_void_p), ("count_dev", C.c_void_p)]
 
 
 class ViewJob(C.Structure):
@@ -142,6 +151,7 @@ SIGNATURES = {
     "bz_pyramid_reference_extent": (None, [_U32, _U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
     "bz_pyramid_build_batch": (_I32, [_VP, C.POINTER(PyramidJob), _U32, _VP]),
     "bz_cull": (_I32, [_VP, C.POINTER(CullArgs), _VP]),
+    "bz_cull_instanced": (_I32, [_VP, C.POINTER(InstancedCullArgs), _VP]),
     "bz_cull_clusters": (_I32, [_VP, C.POINTER(ClusterCullArgs), _VP]),
     "bz_cull_batch": (_I32, [_VP, C.POINTER(ViewJob), _U32, _VP]),
     "bz_check": (_I32, [_VP, _VP]),

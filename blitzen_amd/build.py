@@ -18,8 +18,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libbzcull.so"
 
-SOURCES = ["cull.hip", "pyramid.hip", "cluster.hip", "capi.cpp", "camera.cpp", "synth.cpp", "autotune.cpp",
-           "server.cpp"]
+SOURCES = ["cull.hip", "pyramid.hip", "cluster.hip", "instanced.hip", "capi.cpp", "camera.cpp", "synth.cpp",
+           "autotune.cpp", "server.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

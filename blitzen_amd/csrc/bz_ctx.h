@@ -91,6 +91,8 @@ struct StreamScratch {
     uint32_t pjobs_cap = 0;
     hipEvent_t upload_done = nullptr; // last job-table upload from the pinned buffers
     ClusterScratch xs{};              // cluster pass
+    uint32_t* inst_hist = nullptr;    // LOD-instanced output: chunk x LOD histogram / list offsets
+    uint64_t inst_hist_cap = 0;
     ~StreamScratch();
 };
 

@@ -80,6 +80,23 @@ struct FrameCacheDev {
 // survivors there.
 hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
                        uint32_t per_cu, hipStream_t st, const FrameCacheDev* cache = nullptr);
+// The decide kernel alone: per tile, the emitting objects' (objectId, LOD) in ascending objectId and their count.
+hipError_t launch_decide_only(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items,
+                              uint32_t n_cu, uint32_t per_cu, hipStream_t st);
+
+// LOD-instanced output from a decided job (tiles of tile_size entries in s): per LOD, the ascending object ids
+// (at most per_lod) at lod * per_lod, the counters {lod * per_lod, exact count}, and one 32-byte command per LOD
+// with instances, in ascending LOD order, plus their number.
+struct InstancedJob {
+    const uint2* lod_draw;        // [n_lods] indexCount, firstIndex
+    uint32_t* instance_ids;       // [n_lods * per_lod]
+    uint32_t* counters;           // [n_lods] x 2
+    uint32_t* cmds;               // [capacity] x 8
+    uint32_t* count;
+    uint32_t n_lods, per_lod, capacity, n_tiles, tile_size, tiles_per_chunk, n_chunks, pad;
+};
+hipError_t launch_instanced(const InstancedJob& job, const CompactScratch& s, uint32_t* hist, hipStream_t st);
+
 // LATE pass from a frame cache recorded with the same tile size (block x items) over the same range.
 hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
                               uint32_t items, hipStream_t st);

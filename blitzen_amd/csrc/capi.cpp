@@ -19,6 +19,8 @@
 
 using namespace bz;
 
+static_assert(sizeof(bz_IndirectDrawCmd32) == 32, "IndirectDrawCmd is 32 bytes (dx12Data.h:284-296)");
+static_assert(sizeof(bz_LodInstanceCounter) == 8, "LodInstanceCounter is 8 bytes");
 static_assert(sizeof(bz_RenderObject) == 8, "RenderObject layout");
 static_assert(sizeof(bz_MeshTransform) == 32, "MeshTransform layout");
 static_assert(sizeof(bz_PrimitiveSurface) == 32, "PrimitiveSurface layout");
@@ -82,7 +84,8 @@ Pyramid::~Pyramid()
 
 StreamScratch::~StreamScratch()
 {
-    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err, xs.ccounts, xs.rbase, xs.scount, xs.survivors };
+    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err, xs.ccounts, xs.rbase, xs.scount, xs.survivors,
+                   inst_hist };
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (jobs_host) (void)hipHostFree(jobs_host);
@@ -854,6 +857,66 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
     }
     hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "cull launch");
+    return BZ_OK;
+}
+
+// ---------------------------------------------------------------------------- LOD-instanced cull
+bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* stream)
+{
+    if (!ctx || !a || !a->view) return BZ_E_INVALID;
+    Scene* s = ctx->scene(a->scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (!a->instances_per_lod) return ctx->fail(BZ_E_INVALID, "instances_per_lod must be > 0");
+    if ((uint64_t)s->n_lods * a->instances_per_lod > 0xffffffffull)
+        return ctx->fail(BZ_E_INVALID, "n_lods * instances_per_lod exceeds 2^32 - 1");
+    if (!a->instance_ids_dev || !a->counters_dev || !a->count_dev || (a->capacity && !a->cmds_dev))
+        return ctx->fail(BZ_E_INVALID, "NULL output buffer");
+    if ((uintptr_t)a->cmds_dev % 16u) return ctx->fail(BZ_E_INVALID, "cmds_dev must be 16-byte aligned");
+    const bz_tuning& t = ctx->tune[0];
+    CullJob job;
+    bz_status st_ = ctx->make_job(PASS_PLAIN, 0, a->scene, 0, *a->view, 0, 0, nullptr, nullptr, nullptr, a->count_dev,
+                                  0, t.block, t.items, &job);
+    if (st_) return st_;
+    hipStream_t st = (hipStream_t)stream;
+    InstancedJob ij;
+    std::memset(&ij, 0, sizeof ij);
+    ij.lod_draw = s->lod_draw;
+    ij.instance_ids = a->instance_ids_dev;
+    ij.counters = reinterpret_cast<uint32_t*>(a->counters_dev);
+    ij.cmds = reinterpret_cast<uint32_t*>(a->cmds_dev);
+    ij.count = a->count_dev;
+    ij.n_lods = s->n_lods;
+    ij.per_lod = a->instances_per_lod;
+    ij.capacity = a->capacity;
+    ij.tile_size = t.block * t.items;
+    if (s->n_lods == 0) {
+        hipError_t e = launch_fill_u32(a->count_dev, 0u, 1, st);
+        return e == hipSuccess ? BZ_OK : ctx->hip_fail(e, "count fill");
+    }
+    StreamScratch* sc = nullptr;
+    st_ = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * t.block * t.items, &sc);
+    if (st_) return st_;
+    if (job.n) {
+        // chunks of decided tiles: the histogram holds n_chunks x n_lods counters (at most ~16 M)
+        const uint32_t target = std::max<uint32_t>(1u, std::min<uint32_t>(1024u, (1u << 24) / s->n_lods));
+        ij.tiles_per_chunk = (job.n_tiles + target - 1) / target;
+        ij.n_chunks = (job.n_tiles + ij.tiles_per_chunk - 1) / ij.tiles_per_chunk;
+        ij.n_tiles = job.n_tiles;
+        const uint64_t need = (uint64_t)ij.n_chunks * s->n_lods;
+        if (sc->inst_hist_cap < need) {
+            hipError_t e = hipStreamSynchronize(st);
+            if (sc->inst_hist) (void)hipFree(sc->inst_hist);
+            sc->inst_hist = nullptr;
+            sc->inst_hist_cap = 0;
+            if (e == hipSuccess) e = hipMalloc((void**)&sc->inst_hist, need * 4);
+            if (e != hipSuccess) return ctx->hip_fail(e, "instancing histogram");
+            sc->inst_hist_cap = need;
+        }
+        hipError_t e = launch_decide_only(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "decide launch");
+    }
+    hipError_t e = launch_instanced(ij, sc->cs, sc->inst_hist, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "instanced launch");
     return BZ_OK;
 }
 

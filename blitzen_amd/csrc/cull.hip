@@ -824,7 +824,7 @@ void launch_mode(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* c
 
 template <int BLOCK, int ITEMS>
 hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n_cu, uint32_t per_cu, hipStream_t st,
-                         const FrameCacheDev* fc)
+                         const FrameCacheDev* fc, bool with_emit = true)
 {
     const bool h = job.hiz != 0;
     if (job.pass == PASS_EARLY && fc) {
@@ -840,7 +840,7 @@ hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n
         else launch_mode<BLOCK, ITEMS, MODE_OTHER, false>(job, n_cu, per_cu, s.counts, s.staging, st);
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !with_emit) return e;
     hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
                        (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts, (const uint2*)s.staging);
     return hipGetLastError();
@@ -870,6 +870,16 @@ hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t blo
 {
 #define X(B, I) \
     if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st, cache);
+    BZ_TUNING_TABLE(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_decide_only(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items,
+                              uint32_t n_cu, uint32_t per_cu, hipStream_t st)
+{
+#define X(B, I) \
+    if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st, nullptr, false);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;

@@ -202,6 +202,15 @@ class Context:
                               capacity=capacity)
         self.check(lib().bz_cull_clusters(self.handle, C.byref(a), stream), "bz_cull_clusters")
 
+    def cull_instanced(self, scene: int, view: CameraViewData, per_lod: int, ids: DeviceBuffer, counters: DeviceBuffer,
+                       cmds: DeviceBuffer, count: DeviceBuffer, capacity: int | None = None, stream=None):
+        if capacity is None:
+            capacity = cmds.nbytes // 32
+        a = A.InstancedCullArgs(scene=scene, view=C.pointer(view), instances_per_lod=per_lod, capacity=capacity,
+                                instance_ids_dev=ids.ptr, counters_dev=counters.ptr, cmds_dev=cmds.ptr,
+                                count_dev=count.ptr)
+        self.check(lib().bz_cull_instanced(self.handle, C.byref(a), stream), "bz_cull_instanced")
+
     def read_commands(self, cmds: DeviceBuffer, count: DeviceBuffer, capacity: int | None = None, stream=None):
         n = int(count.download(np.uint32, 1, stream)[0])
         cap = cmds.nbytes // 24 if capacity is None else capacity

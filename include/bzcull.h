@@ -267,6 +267,42 @@ typedef struct bz_cluster_cull_args {
 BZ_API bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* args, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
+ * LOD-instanced cull — replaces the D3D12 renderer's instanced path: drawInstCountReset.cs.hlsl,
+ * drawInstCull.cs.hlsl and drawInstCmd.cs.hlsl, host DrawInstanceCullPass (dx12Draw.cpp:340-413).
+ * Frustum + LOD per object (the BZ_PASS_PLAIN decision); each visible object joins the instance list of its
+ * LOD; one instanced command per LOD with instances, in ascending LOD order. Deterministic: every list holds
+ * its objects in ascending objectId; a list keeps at most instances_per_lod of them (the reference's
+ * Ce_MaxInstanceCountPerLOD, blitzenEngine.h:65, overflows into the next list instead) while its counter keeps
+ * the exact count, and the command draws min(count, instances_per_lod) instances.
+ * ------------------------------------------------------------------------------------------- */
+/* LodInstanceCounter — renderingResourcesTypes.h:98-102 */
+typedef struct bz_LodInstanceCounter {
+    uint32_t instanceOffset;   /* lod * instances_per_lod */
+    uint32_t instanceCount;    /* exact number of visible objects at this LOD */
+} bz_LodInstanceCounter;
+/* IndirectDrawCmd — dx12Data.h:284-296 == cullBuffers.hlsl DrawCmd (32 B) */
+typedef struct bz_IndirectDrawCmd32 {
+    uint32_t objId;            /* the LOD's instanceOffset (root constant of the instanced draw) */
+    uint32_t indexCount;
+    uint32_t instanceCount;
+    uint32_t firstIndex;
+    int32_t vertexOffset;      /* 0 */
+    uint32_t firstInstance;    /* 0 */
+    uint32_t padding0, padding1;
+} bz_IndirectDrawCmd32;
+typedef struct bz_instanced_cull_args {
+    bz_scene scene;
+    const bz_CameraViewData* view;
+    uint32_t instances_per_lod;
+    uint32_t capacity;                        /* commands */
+    uint32_t* instance_ids_dev;               /* [n_lods * instances_per_lod] object ids */
+    bz_LodInstanceCounter* counters_dev;      /* [n_lods] */
+    bz_IndirectDrawCmd32* cmds_dev;
+    uint32_t* count_dev;                      /* number of commands (LODs with instances) */
+} bz_instanced_cull_args;
+BZ_API bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* args, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
  * Batched multi-view cull: many views (cameras, shadow cascades) of uneven object counts in one
  * persistent launch (late-pass semantics per view, each with its own pyramid, visibility and outputs).
  * ------------------------------------------------------------------------------------------- */

@@ -24,6 +24,7 @@
  *                         261-278, 312-372, 497-534, 559-570; blitMLTypes.h:143-193)
  *   orc_cluster_*         PreClusterDrawCull.comp.glsl:13-47, InitialClusterCull.comp.glsl:12-55, plus the
  *                         cluster tests this library defines (the reference has none).
+ *   orc_cull_instanced    drawInstCull.cs.hlsl + drawInstCmd.cs.hlsl (LOD-instanced output, deterministic order)
  *
  * Points the reference leaves to fixed-function hardware are settled by explicit rules, shared with the
  * HIP kernels:
@@ -322,6 +323,44 @@ ORC_EXPORT int64_t orc_cull(const OrcObject* objects, uint32_t n_objects, const 
     int64_t total = (int64_t)chunk_count[nt];
     free(chunk_count);
     return total;
+}
+
+/* LOD-instanced output (drawInstCountReset / drawInstCull / drawInstCmd .cs.hlsl; host dx12Draw.cpp:340-413) with
+ * this library's deterministic rules: objects visit in ascending index; each visible object (the PLAIN decision,
+ * as drawInstCull's frustum + LODSelection) is appended to its LOD's list while the list holds fewer than per_lod;
+ * the counter counts every one. Commands, one per LOD with instances, in ascending LOD:
+ * {lod * per_lod, indexCount, min(count, per_lod), firstIndex, 0, 0, 0, 0}. Returns the number of commands. */
+ORC_EXPORT int64_t orc_cull_instanced(const OrcObject* objects, uint32_t n_objects, const OrcTransform* transforms,
+                                      uint32_t n_transforms, const OrcSurface* surfaces, uint32_t n_surfaces,
+                                      const OrcLod* lods, uint32_t n_lods, const OrcView* view, uint32_t per_lod,
+                                      uint32_t* instance_ids, uint32_t* counters, uint32_t* cmds, uint32_t capacity)
+{
+    OrcScene s = { objects, n_objects, transforms, n_transforms, surfaces, n_surfaces, lods, n_lods };
+    for (uint32_t l = 0; l < n_lods; ++l) {
+        counters[2 * l] = l * per_lod;
+        counters[2 * l + 1] = 0;
+    }
+    for (uint32_t i = 0; i < n_objects; ++i) {
+        uint32_t la = 0, vn = 0;
+        if (!cull_object(&s, view, NULL, ORC_PLAIN, 0, i, 0, &la, &vn)) continue;
+        uint32_t k = counters[2 * la + 1]++;
+        if (k < per_lod) instance_ids[(size_t)la * per_lod + k] = i;
+    }
+    int64_t n = 0;
+    for (uint32_t l = 0; l < n_lods; ++l) {
+        uint32_t c = counters[2 * l + 1];
+        if (!c) continue;
+        if ((uint64_t)n < capacity) {
+            uint32_t* d = cmds + (size_t)n * 8;
+            d[0] = l * per_lod;
+            d[1] = lods[l].indexCount;
+            d[2] = c < per_lod ? c : per_lod;
+            d[3] = lods[l].firstIndex;
+            d[4] = d[5] = d[6] = d[7] = 0;
+        }
+        ++n;
+    }
+    return n;
 }
 
 /* ================================================================================================

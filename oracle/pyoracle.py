@@ -70,6 +70,8 @@ def load():
         lib.orc_cluster_visible.restype = i32
         lib.orc_cluster_visible.argtypes = [vp, vp, vp, vp, u32]
         lib.orc_threads_available.restype = i32
+        lib.orc_cull_instanced.restype = C.c_int64
+        lib.orc_cull_instanced.argtypes = [vp, u32, vp, u32, vp, u32, vp, u32, vp, u32, vp, vp, vp, u32]
         _lib = lib
     return _lib
 
@@ -185,3 +187,19 @@ def cluster_cull(recs: np.ndarray, objects, transforms, clusters, view_bytes: by
                                 C.byref(pyr) if pyr is not None else None, tests, p(cmds), cap)
     k = min(n, cap)
     return cmds[:k * 6].reshape(k, 6), int(n)
+
+
+def cull_instanced(scene, view_bytes: bytes, per_lod: int, capacity: int | None = None):
+    """Returns (instance_ids (n_lods, per_lod) uint32 with unwritten slots = 0xFFFFFFFF, counters (n_lods, 2),
+    commands (k, 8) uint32, number of commands)."""
+    objects, transforms, surfaces, lods = scene
+    nl = len(lods)
+    ids = np.full(nl * per_lod, 0xFFFFFFFF, np.uint32)
+    counters = np.zeros(nl * 2, np.uint32)
+    cap = nl if capacity is None else capacity
+    cmds = np.zeros(max(cap, 1) * 8, np.uint32)
+    vb = np.frombuffer(view_bytes, np.uint8).copy()
+    n = load().orc_cull_instanced(p(objects), len(objects), p(transforms), len(transforms), p(surfaces),
+                                  len(surfaces), p(lods), nl, p(vb), per_lod, p(ids), p(counters), p(cmds), cap)
+    k = min(n, cap)
+    return ids.reshape(nl, per_lod), counters.reshape(nl, 2), cmds[:k * 8].reshape(k, 8), int(n)

@@ -199,3 +199,31 @@ def test_capacity_and_ranges(scene):
         np.testing.assert_array_equal(sub, sel)
     with pytest.raises(ValueError):
         run(scene, vb, A.PASS_PLAIN, begin=n - 1, count=2)
+
+
+@pytest.mark.parametrize("per_lod", [100000, 50])
+def test_instanced_is_plain_grouped_by_lod(scene, per_lod):
+    """The LOD-instanced output equals the PLAIN pass's commands grouped by LOD (numpy grouping, ascending
+    objectId within a LOD), lists capped at per_lod with exact counters, one command per non-empty LOD."""
+    vb = H.view_bytes(H.view())
+    cmds, n, lod = orc.cull(scene, vb, A.PASS_PLAIN, want_lod=True)
+    ids, counters, icmds, k = orc.cull_instanced(scene, vb, per_lod)
+    lods = scene[3]
+    nl = len(lods)
+    np.testing.assert_array_equal(counters[:, 0], np.arange(nl) * per_lod)
+    np.testing.assert_array_equal(counters[:, 1], np.bincount(lod, minlength=nl))
+    for l in np.unique(lod):
+        members = cmds[lod == l, 0]  # ascending objectId already
+        m = min(len(members), per_lod)
+        np.testing.assert_array_equal(ids[l, :m], members[:m])
+        assert (ids[l, m:] == 0xFFFFFFFF).all()
+    used = np.unique(lod)
+    assert k == len(used)
+    np.testing.assert_array_equal(icmds[:, 0], used * per_lod)
+    np.testing.assert_array_equal(icmds[:, 1], lods["indexCount"][used])
+    np.testing.assert_array_equal(icmds[:, 2], np.minimum(np.bincount(lod, minlength=nl)[used], per_lod))
+    np.testing.assert_array_equal(icmds[:, 3], lods["firstIndex"][used])
+    np.testing.assert_array_equal(icmds[:, 4:], 0)
+    _, _, part, k2 = orc.cull_instanced(scene, vb, per_lod, capacity=3)
+    assert k2 == k and len(part) == 3
+    np.testing.assert_array_equal(part, icmds[:3])
