@@ -122,6 +122,8 @@ CULL_EARLY_HIZ = 0x1
 CULL_FRAME_CACHE = 0x2
 CLUSTER_FRUSTUM, CLUSTER_HIZ, CLUSTER_CONE = 0x1, 0x2, 0x4
 SERVER_RELEASE_ON_COMPLETE = 0x1
+EXTSEM_BINARY = 0
+EXTSEM_TIMELINE = 1
 
 _VP = C.c_void_p
 _U32, _U64, _F32, _I32 = C.c_uint32, C.c_uint64, C.c_float, C.c_int32
@@ -152,6 +154,13 @@ SIGNATURES = {
     "bz_pyramid_build_batch": (_I32, [_VP, C.POINTER(PyramidJob), _U32, _VP]),
     "bz_cull": (_I32, [_VP, C.POINTER(CullArgs), _VP]),
     "bz_cull_instanced": (_I32, [_VP, C.POINTER(InstancedCullArgs), _VP]),
+    "bz_extmem_import": (_I32, [_VP, C.c_int, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "bz_extmem_map": (_I32, [_VP, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "bz_extmem_release": (_I32, [_VP, C.c_uint64]),
+    "bz_extsem_import": (_I32, [_VP, C.c_int, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "bz_extsem_wait": (_I32, [_VP, C.c_uint64, C.c_uint64, _VP]),
+    "bz_extsem_signal": (_I32, [_VP, C.c_uint64, C.c_uint64, _VP]),
+    "bz_extsem_release": (_I32, [_VP, C.c_uint64]),
     "bz_cull_clusters": (_I32, [_VP, C.POINTER(ClusterCullArgs), _VP]),
     "bz_cull_batch": (_I32, [_VP, C.POINTER(ViewJob), _U32, _VP]),
     "bz_check": (_I32, [_VP, _VP]),

@@ -107,6 +107,23 @@ struct Graph {
 
 struct Server;  // server.cpp
 
+// Memory imported from another API (a Vulkan buffer exported as a file descriptor) and the device pointers mapped
+// onto it. `fd` is this library's duplicate of the caller's descriptor while the runtime has not taken it over.
+struct ExtMem {
+    hipExternalMemory_t mem = nullptr;
+    int fd = -1;
+    uint64_t size = 0;
+    std::vector<void*> maps;
+    ~ExtMem();  // interop.cpp
+};
+
+struct ExtSem {
+    hipExternalSemaphore_t sem = nullptr;
+    int fd = -1;
+    uint32_t kind = 0;
+    ~ExtSem();  // interop.cpp
+};
+
 struct Ctx {
     int device = 0;
     uint32_t n_cu = 0;
@@ -118,6 +135,8 @@ struct Ctx {
     std::unordered_map<uint64_t, std::unique_ptr<Scene>> scenes;
     std::unordered_map<uint64_t, std::unique_ptr<Pyramid>> pyramids;
     std::unordered_map<uint64_t, std::unique_ptr<Graph>> graphs;
+    std::unordered_map<uint64_t, std::unique_ptr<ExtMem>> extmems;
+    std::unordered_map<uint64_t, std::unique_ptr<ExtSem>> extsems;
     std::unordered_map<uintptr_t, std::unique_ptr<StreamScratch>> scratch;
     bz_tuning tune[2] = {};
     std::unique_ptr<Server> server;

@@ -63,6 +63,18 @@ class DeviceBuffer:
             pass
 
 
+class MappedBuffer(DeviceBuffer):
+    """Device pointer onto imported external memory; unmapped by Context.extmem_release, not by free()."""
+
+    def __init__(self, ctx: "Context", ptr: int, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = ptr
+
+    def free(self):
+        self.ptr = None
+
+
 class Context:
     def __init__(self, device: int = 0):
         cfg = A.Config(device=device)
@@ -210,6 +222,34 @@ class Context:
                                 instance_ids_dev=ids.ptr, counters_dev=counters.ptr, cmds_dev=cmds.ptr,
                                 count_dev=count.ptr)
         self.check(lib().bz_cull_instanced(self.handle, C.byref(a), stream), "bz_cull_instanced")
+
+    # ------------------------------------------------------------------ interop (Vulkan-exported fds)
+    def extmem_import(self, fd: int, size: int) -> int:
+        h = C.c_uint64()
+        self.check(lib().bz_extmem_import(self.handle, fd, size, C.byref(h)), "bz_extmem_import")
+        return h.value
+
+    def extmem_map(self, mem: int, offset: int, size: int) -> MappedBuffer:
+        p = C.c_void_p()
+        self.check(lib().bz_extmem_map(self.handle, mem, offset, size, C.byref(p)), "bz_extmem_map")
+        return MappedBuffer(self, p.value, size)
+
+    def extmem_release(self, mem: int):
+        self.check(lib().bz_extmem_release(self.handle, mem), "bz_extmem_release")
+
+    def extsem_import(self, fd: int, kind: int = A.EXTSEM_BINARY) -> int:
+        h = C.c_uint64()
+        self.check(lib().bz_extsem_import(self.handle, fd, kind, C.byref(h)), "bz_extsem_import")
+        return h.value
+
+    def extsem_wait(self, sem: int, value: int = 0, stream=None):
+        self.check(lib().bz_extsem_wait(self.handle, sem, value, stream), "bz_extsem_wait")
+
+    def extsem_signal(self, sem: int, value: int = 0, stream=None):
+        self.check(lib().bz_extsem_signal(self.handle, sem, value, stream), "bz_extsem_signal")
+
+    def extsem_release(self, sem: int):
+        self.check(lib().bz_extsem_release(self.handle, sem), "bz_extsem_release")
 
     def read_commands(self, cmds: DeviceBuffer, count: DeviceBuffer, capacity: int | None = None, stream=None):
         n = int(count.download(np.uint32, 1, stream)[0])

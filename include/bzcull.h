@@ -429,6 +429,31 @@ typedef struct bz_server_stats {
 BZ_API bz_status bz_server_get_stats(bz_ctx* ctx, bz_server_stats* out);
 
 /* ---------------------------------------------------------------------------------------------
+ * Vulkan interop (Linux). The renderer keeps owning the cull outputs and the depth attachment as Vulkan memory
+ * (the command + count buffers vkCmdDrawIndexedIndirectCount reads, vulkanDraw.cpp:470-471; the depth the pyramid
+ * is built from, vulkanDraw.cpp:554-575) and exports them with VK_KHR_external_memory_fd (vkGetMemoryFdKHR, an
+ * opaque fd: a dma-buf on amdgpu). The mapped device pointers go into bz_cull / bz_pyramid_build like any other.
+ * Depth must be linear: copy the attachment into an exported buffer (vkCmdCopyImageToBuffer, D32_SFLOAT, tightly
+ * packed rows = float[h][w]). Semaphores exported with VK_KHR_external_semaphore_fd order the two queues.
+ * The caller's descriptor is never consumed: the library imports a duplicate, so the caller may close its own.
+ * ------------------------------------------------------------------------------------------- */
+typedef uint64_t bz_extmem;
+typedef uint64_t bz_extsem;
+/* Imports `size` bytes (the Vulkan allocation's size) of external memory from `fd`. */
+BZ_API bz_status bz_extmem_import(bz_ctx* ctx, int fd, uint64_t size, bz_extmem* out);
+/* Device pointer to bytes [offset, offset + size) of the import, valid until bz_extmem_release. */
+BZ_API bz_status bz_extmem_map(bz_ctx* ctx, bz_extmem mem, uint64_t offset, uint64_t size, void** dev_ptr);
+/* Synchronizes the device, unmaps every pointer of `mem` and releases the import. */
+BZ_API bz_status bz_extmem_release(bz_ctx* ctx, bz_extmem mem);
+#define BZ_EXTSEM_BINARY   0u /* VkSemaphore of VK_SEMAPHORE_TYPE_BINARY, opaque fd   */
+#define BZ_EXTSEM_TIMELINE 1u /* VkSemaphore of VK_SEMAPHORE_TYPE_TIMELINE, opaque fd */
+BZ_API bz_status bz_extsem_import(bz_ctx* ctx, int fd, uint32_t kind, bz_extsem* out);
+/* Stream-ordered wait for / signal of the semaphore (`value` is the timeline value; ignored for binary). */
+BZ_API bz_status bz_extsem_wait(bz_ctx* ctx, bz_extsem sem, uint64_t value, void* stream);
+BZ_API bz_status bz_extsem_signal(bz_ctx* ctx, bz_extsem sem, uint64_t value, void* stream);
+BZ_API bz_status bz_extsem_release(bz_ctx* ctx, bz_extsem sem);
+
+/* ---------------------------------------------------------------------------------------------
  * Seeded synthetic workloads (SplitMix64 streams), identical on every host.
  * ------------------------------------------------------------------------------------------- */
 typedef struct bz_synth_scene_cfg {
