@@ -557,14 +557,33 @@ struct EmitSmem {
     uint32_t job;
 };
 
-// Writes the commands of tiles [t0, t1) of one job (t1 - t0 <= kEmitTiles). One round of loads gets the counts of
-// the job's earlier tiles (reduced to the output base) and of the workgroup's own tiles; then wave w writes tile
-// t0 + w's entries in order, kEmitUnroll per lane with all their loads in flight.
+// A workgroup barrier for LDS only: waits for this wave's LDS operations (lgkmcnt(0)) but not for its global loads in
+// flight (vmcnt and expcnt left at their maxima), then s_barrier.
+__device__ __forceinline__ void block_lds_sync()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Writes the commands of tiles [t0, t1) of one job (t1 - t0 <= kEmitTiles; tile_size >= 64 * kEmitUnroll). One round
+// of loads gets the counts of the job's earlier tiles (reduced to the output base), the workgroup's own counts and,
+// before any count is known, the first 64 * kEmitUnroll staging entries of each wave's tile (in bounds: every tile
+// owns tile_size slots); each wave gathers its entries' LOD draw data while the block scan of the counts runs, then
+// writes tile t0 + w's commands in order. Longer tiles continue in rounds of kEmitUnroll entries per lane.
 __device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32_t t1, uint32_t tile_size,
                                            const uint32_t* counts, const uint2* staging, EmitSmem& sm)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t nt = t1 - t0;
+    const uint32_t tw = t0 + (wave < nt ? wave : 0u);  // this wave's tile (clamped for waves without one)
+    const uint2* stage = staging + (size_t)tw * tile_size;
+    const uint2* lod_draw = J.scene.lod_draw;
+    uint2 ent[kEmitUnroll], d[kEmitUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kEmitUnroll; ++u) ent[u] = stage[u * 64u + lane];  // speculative: checked against c below
+    const uint32_t c = wave < nt ? counts[tw] : 0u;
     // the job's earlier tiles: every load of a thread issued before the first add (16 per pass)
     uint32_t part = 0;
     const uint32_t own = tid < nt ? counts[t0 + tid] : 0u;
@@ -579,6 +598,9 @@ __device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32
         for (int j = 0; j < 16; ++j) part += v[j];
     }
 #pragma unroll
+    for (uint32_t u = 0; u < kEmitUnroll; ++u)
+        d[u] = u * 64u + lane < c ? lod_draw[ent[u].y] : make_uint2(0u, 0u);  // in flight through the scan
+#pragma unroll
     for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
     if (lane == 0) sm.red[wave] = part;
     if (wave == 0) {
@@ -591,25 +613,23 @@ __device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32
         if (lane < kEmitTiles) sm.cnt[lane + 1] = v;
         if (lane == 0) sm.cnt[0] = 0;
     }
-    __syncthreads();
+    block_lds_sync();
     uint32_t prefix = 0;
 #pragma unroll
     for (uint32_t w = 0; w < kEmitBlock / 64u; ++w) prefix += sm.red[w];
     if (tid == 0 && t1 == J.tile_start + J.n_tiles) *J.count = prefix + sm.cnt[nt];
     if (wave >= nt) return;
-    const uint32_t c = sm.cnt[wave + 1] - sm.cnt[wave];
     const uint32_t base = prefix + sm.cnt[wave];
-    const uint2* stage = staging + (size_t)(t0 + wave) * tile_size;
-    const uint2* lod_draw = J.scene.lod_draw;
     for (uint32_t e0 = 0; e0 < c; e0 += 64u * kEmitUnroll) {
-        uint2 ent[kEmitUnroll], d[kEmitUnroll];
+        if (e0) {
 #pragma unroll
-        for (uint32_t u = 0; u < kEmitUnroll; ++u) {
-            const uint32_t e = e0 + u * 64u + lane;
-            ent[u] = stage[e < c ? e : 0u];  // objectId, absolute LOD (entry 0 exists whenever c > 0)
+            for (uint32_t u = 0; u < kEmitUnroll; ++u) {
+                const uint32_t e = e0 + u * 64u + lane;
+                ent[u] = stage[e < c ? e : 0u];  // objectId, absolute LOD (entry 0 exists whenever c > 0)
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kEmitUnroll; ++u) d[u] = lod_draw[ent[u].y];
         }
-#pragma unroll
-        for (uint32_t u = 0; u < kEmitUnroll; ++u) d[u] = lod_draw[ent[u].y];
 #pragma unroll
         for (uint32_t u = 0; u < kEmitUnroll; ++u) {
             const uint32_t e = e0 + u * 64u + lane;
@@ -680,6 +700,18 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
     const uint32_t base = st * kSuperTile;  // first object of the super-tile, relative to the job's range
     const uint32_t nt = J.n - base < kSuperTile ? J.n - base : kSuperTile;
     const uint32_t obj0 = J.begin + base;
+    // speculative first round, issued before the counts are known: entries [0, WIN) of every recorded tile, survivor
+    // slot g = k * BLOCK + tid -> (tile g / WIN, entry g % WIN), i.e. ascending objectId
+    constexpr uint32_t WIN = BLOCK * U / S;
+    float4 sp[U];
+    uint2 mt[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k) {
+        const uint32_t g = k * BLOCK + tid, rt = st * S + g / WIN;
+        const size_t slot = (size_t)(rt < n_rec_tiles ? rt : 0u) * TILE + g % WIN;
+        sp[k] = fc.sph[slot];
+        mt[k] = fc.meta[slot];
+    }
     uint32_t cnt = 0;
     if (tid < S) {
         const uint32_t t = st * S + tid;
@@ -705,24 +737,9 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
     __syncthreads();
     const uint32_t total = s_cnt[S];
     uint2* stage = staging + (size_t)st * kSuperTile;
-    // survivors in rounds of BLOCK x U (normally one round); g = k * BLOCK + tid is the survivor's rank in the
-    // super-tile, i.e. ascending objectId
-    for (uint32_t g0 = 0; g0 < total; g0 += BLOCK * U) {
-        float4 sp[U];
-        uint2 mt[U];
-        bool have[U];
-#pragma unroll
-        for (uint32_t k = 0; k < U; ++k) {
-            const uint32_t g = g0 + k * BLOCK + tid;
-            have[k] = g < total;
-            uint32_t j = 0;  // recorded tile holding survivor g
-#pragma unroll
-            for (uint32_t q = 1; q < S; ++q) j += g >= s_cnt[q] ? 1u : 0u;
-            const uint32_t e = have[k] ? g - s_cnt[j] : 0u;
-            const size_t slot = (size_t)(st * S + (have[k] ? j : 0u)) * TILE + e;  // in the cache region
-            sp[k] = fc.sph[slot];
-            mt[k] = fc.meta[slot];
-        }
+    // one round of up to BLOCK x U survivors: Hi-Z test, visibility bits, commands in (k, wave, lane) order after the
+    // earlier rounds
+    auto round = [&](const float4 (&sp)[U], const uint2 (&mt)[U], const bool (&have)[U]) {
         bool emit[U];
         uint32_t lod_abs[U];
 #pragma unroll
@@ -742,7 +759,6 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
                 lod_abs[k] = mt[k].y & 0x7fffffffu;
             }
         }
-        // commands of this round in survivor order: ranks in (k, wave, lane) order after the earlier rounds
         uint32_t rank[U];
 #pragma unroll
         for (uint32_t k = 0; k < U; ++k) {
@@ -763,6 +779,38 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
         for (uint32_t k = 0; k < U; ++k)
             if (emit[k]) stage[s_woff[k][wave] + rank[k]] = make_uint2(mt[k].x, lod_abs[k]);
         __syncthreads();  // s_woff is rewritten by the next round
+    };
+    bool fast = true;  // every recorded tile's survivors fit its window: the speculative round is the whole pass
+#pragma unroll
+    for (uint32_t q = 0; q < S; ++q) fast = fast && s_cnt[q + 1] - s_cnt[q] <= WIN;
+    if (fast) {
+        bool have[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t g = k * BLOCK + tid, j = g / WIN;
+            have[k] = g % WIN < s_cnt[j + 1] - s_cnt[j];
+        }
+        round(sp, mt, have);
+    } else {
+        // survivors in rounds of BLOCK x U; g = k * BLOCK + tid is the survivor's rank in the super-tile
+        for (uint32_t g0 = 0; g0 < total; g0 += BLOCK * U) {
+            float4 sp2[U];
+            uint2 mt2[U];
+            bool have[U];
+#pragma unroll
+            for (uint32_t k = 0; k < U; ++k) {
+                const uint32_t g = g0 + k * BLOCK + tid;
+                have[k] = g < total;
+                uint32_t j = 0;  // recorded tile holding survivor g
+#pragma unroll
+                for (uint32_t q = 1; q < S; ++q) j += g >= s_cnt[q] ? 1u : 0u;
+                const uint32_t e = have[k] ? g - s_cnt[j] : 0u;
+                const size_t slot = (size_t)(st * S + (have[k] ? j : 0u)) * TILE + e;  // in the cache region
+                sp2[k] = fc.sph[slot];
+                mt2[k] = fc.meta[slot];
+            }
+            round(sp2, mt2, have);
+        }
     }
 #pragma unroll
     for (uint32_t k = 0; k < OPT; ++k) {

@@ -189,3 +189,31 @@ def test_frame_cache_keyed_on_visibility_input(ctx, pyramid):
     np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
     other.free()
     ctx.scene_destroy(h)
+
+
+@pytest.mark.parametrize("tuning", [(512, 2), (256, 2), (512, 4)])
+def test_frame_cache_dense_survivors(ctx, pyramid, tuning):
+    """Recorded tiles where most objects pass the frustum (more survivors than the cached late pass's first-round
+    window, and more commands per tile than one emit round) next to sparse ones: outputs still equal the oracle's."""
+    n = 120000
+    sc = H.c1_scene(n)
+    xf = sc.transforms.copy()
+    rng = np.random.default_rng(12)
+    dense = (np.arange(n) // 20000) % 2 == 0  # alternating dense and sparse runs of 20000 objects
+    z = rng.uniform(20.0, 400.0, n).astype(np.float32)
+    xf["pos"][dense, 2] = z[dense]
+    xf["pos"][dense, 0] = (rng.uniform(-0.3, 0.3, n).astype(np.float32) * z)[dense]
+    xf["pos"][dense, 1] = (rng.uniform(-0.2, 0.2, n).astype(np.float32) * z)[dense]
+    scene = (sc.objects, xf, sc.surfaces, sc.lods)
+    v = H.view(pyr=(1024, 512))
+    _, n_plain = orc.cull(scene, H.view_bytes(v), A.PASS_PLAIN)
+    assert n_plain > 0.4 * n  # the dense runs really are dense
+    h = ctx.scene_create(*scene)
+    old = ctx.get_tuning(0)
+    ctx.set_tuning(0, *tuning)
+    try:
+        frame(ctx, h, scene, v, pyramid, bz.synth_visibility(13, n, 0.5))
+        frame(ctx, h, scene, v, pyramid, bz.synth_visibility(14, n, 0.05), in_place=False)
+    finally:
+        ctx.set_tuning(0, old.block, old.items, old.grid_per_cu)
+    ctx.scene_destroy(h)
