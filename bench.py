@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-clusters", action="store_true")
     ap.add_argument("--served-seconds", type=float, default=0.5)
+    ap.add_argument("--no-frame-cache", action="store_true",
+                    help="run the C1 passes without BZ_CULL_FRAME_CACHE (the LATE pass reads every object)")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="first copy N MiB device to device 3 times (counter calibration for profiles)")
     args = ap.parse_args()
@@ -310,6 +312,7 @@ def main():
             self.ptr = p
 
     ce, cl = _Cnt(cnt_e_ptr), _Cnt(cnt_l_ptr)
+    fc_flag = 0 if args.no_frame_cache else A.CULL_FRAME_CACHE
 
     def step(k, ev=None):
         h, vin, vout, dbuf, pyr = copies[k % len(copies)]
@@ -318,11 +321,11 @@ def main():
         ctx.pyramid_build(pyr, dbuf.ptr, DEPTH_W, DEPTH_H, stream)
         if ev:
             ev[1].record(stream)
-        ctx.cull(h, v, A.PASS_EARLY, cmds_e, ce, capacity=N_OBJECTS, flags=A.CULL_FRAME_CACHE, vis_in=vin.ptr,
+        ctx.cull(h, v, A.PASS_EARLY, cmds_e, ce, capacity=N_OBJECTS, flags=fc_flag, vis_in=vin.ptr,
                  stream=stream)
         if ev:
             ev[2].record(stream)
-        ctx.cull(h, v, A.PASS_LATE, cmds_l, cl, capacity=N_OBJECTS, pyramid=pyr, flags=A.CULL_FRAME_CACHE,
+        ctx.cull(h, v, A.PASS_LATE, cmds_l, cl, capacity=N_OBJECTS, pyramid=pyr, flags=fc_flag,
                  vis_in=vin.ptr, vis_out=vout.ptr, stream=stream)
         if ev:
             ev[3].record(stream)
@@ -340,16 +343,22 @@ def main():
     counts = cnt.download(np.uint32, 2, stream)
     n_early, n_late = int(counts[0]), int(counts[1])
 
-    # timed region: wall clock over exactly K steps, device synchronized on both sides
+    # timed region: wall clock over exactly K steps, device synchronized on both sides; a HIP event pair on the
+    # launch stream brackets the same K frames (the roofline's average frame duration)
     ctx.sync(stream)
+    e_start, e_end = Event(), Event()
     t_start = time.perf_counter()
+    e_start.record(stream)
     for k in range(args.steps):
         step(k)
+    e_end.record(stream)
     ctx.sync(stream)
     wall = time.perf_counter() - t_start
     ctx.check_kernels(stream)
+    frame_ms = e_start.elapsed_ms(e_end) / args.steps
 
-    # per-kernel durations with HIP events on the launch stream (separate pass, same work)
+    # per-pass durations with HIP events between the passes (separate run of the same work; the markers between
+    # passes add a little time of their own)
     evs = [[Event() for _ in range(4)] for _ in range(args.steps)]
     for k in range(args.steps):
         step(k, evs[k])
@@ -362,7 +371,6 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = N_OBJECTS / (wall / args.steps)
     late_ms = statistics.mean(t_late)
-    frame_ms = statistics.mean(t_frame)
     late_bytes = N_OBJECTS * LATE_BYTES_PER_OBJECT + n_late * CMD_BYTES + 4
     early_bytes = N_OBJECTS * EARLY_BYTES_PER_OBJECT + int(vis0.sum()) * 40 + n_early * CMD_BYTES + 4
     pyr_bytes = DEPTH_W * DEPTH_H * 4 + sum(max(1, PYR_W >> L) * max(1, PYR_H >> L) for L in range(PYR_MIPS)) * 4
@@ -373,9 +381,11 @@ def main():
     traffic = load_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                "kernel": "C1 frame: pyramid_kernel + decide_kernel (EARLY, frame cache) + emit_kernel + "
-                          "late_cached_kernel + emit_kernel",
+                "traffic": traffic.get("bytes_per_launch") if traffic and not args.no_frame_cache else None,
+                "kernel": ("C1 frame: pyramid_kernel + decide_kernel (EARLY) + emit_kernel + decide_kernel (LATE) + "
+                           "emit_kernel" if args.no_frame_cache else
+                           "C1 frame: pyramid_kernel + decide_kernel (EARLY, frame cache) + emit_kernel + "
+                           "late_cached_kernel + emit_kernel"),
                 "algorithmic_bytes": frame_bytes, "avg_us": round(frame_ms * 1e3, 2)}
 
     result = {
@@ -393,11 +403,12 @@ def main():
         "data": "synthetic (seeded)",
         "config": {"workload": "C1: 4,194,304 objects, 256 surfaces x 8 LODs, 1920x1080 depth -> 1024x512 "
                                "11-mip pyramid, visibility Bernoulli(0.5), pyramid + early + late per step",
-                   "objects": N_OBJECTS, "copies_rotated": args.copies},
+                   "objects": N_OBJECTS, "copies_rotated": args.copies, "frame_cache": not args.no_frame_cache},
         "passes_us": {"pyramid": round(statistics.mean(t_pyr) * 1e3, 2),
                       "early": round(statistics.mean(t_early) * 1e3, 2),
                       "late": round(late_ms * 1e3, 2),
-                      "frame_events": round(statistics.mean(t_frame) * 1e3, 2)},
+                      "frame_events": round(statistics.mean(t_frame) * 1e3, 2),
+                      "frame_timed_region": round(frame_ms * 1e3, 2)},
         "pass_gbs": {"pyramid": round(pyr_bytes / (statistics.mean(t_pyr) * 1e-3) / 1e9, 1),
                      "early": round(early_bytes / (statistics.mean(t_early) * 1e-3) / 1e9, 1),
                      "late": round(late_bytes / (late_ms * 1e-3) / 1e9, 1)},
