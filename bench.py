@@ -241,6 +241,70 @@ def bench_clusters(ctx, stream, steps: int, warmup: int):
     return out
 
 
+def bench_next_rows(ctx, stream, scenes, view, transforms, steps: int):
+    """The scope table's next rows at the C1 size: the transparent (PLAIN) and ONPC passes, the LOD-instanced cull
+    and 1% transform streaming. Each is timed by a HIP event pair around `steps` launches on the launch stream,
+    rotating over the resident scene copies (so the 160 MB scene is not L2/MALL-resident between launches)."""
+    import numpy as np
+
+    from blitzen_amd import _capi as A
+    from blitzen_amd import cull as bz
+    from blitzen_amd.hip import Event
+
+    def timed(fn):
+        for k in range(3):
+            fn(k)
+        ctx.sync(stream)
+        e0, e1 = Event(), Event()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for k in range(steps):
+            fn(k)
+        e1.record(stream)
+        ctx.sync(stream)
+        wall = (time.perf_counter() - t0) / steps
+        ctx.check_kernels(stream)
+        return e0.elapsed_ms(e1) / steps, wall
+
+    def row(ms, alg, extra):
+        gbs = alg / (ms * 1e-3) / 1e9
+        return {"us": round(ms * 1e3, 2), "object_culls_per_s": round(N_OBJECTS / (ms * 1e-3), 1),
+                "algorithmic_bytes": alg, "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), **extra}
+
+    out = {}
+    cmds = ctx.alloc(N_OBJECTS * CMD_BYTES)
+    cnt = ctx.alloc(4)
+    for name, pass_ in (("plain", A.PASS_PLAIN), ("onpc", A.PASS_ONPC)):
+        ms, _ = timed(lambda k: ctx.cull(scenes[k % len(scenes)], view, pass_, cmds, cnt, capacity=N_OBJECTS,
+                                         stream=stream))
+        n = int(cnt.download(np.uint32, 1, stream)[0])
+        out[name] = row(ms, N_OBJECTS * 40 + n * CMD_BYTES + 4, {"commands": n})
+    # LOD-instanced: every LOD's list has room for 8192 instances (the largest list here holds a few thousand)
+    n_lods = N_SURFACES * N_LODS
+    per_lod = 8192
+    ids, counters = ctx.alloc(n_lods * per_lod * 4), ctx.alloc(n_lods * 8)
+    icmds, icnt = ctx.alloc(n_lods * 32), ctx.alloc(4)
+    ms, _ = timed(lambda k: ctx.cull_instanced(scenes[k % len(scenes)], view, per_lod, ids, counters, icmds, icnt,
+                                               stream=stream))
+    k_cmds = int(icnt.download(np.uint32, 1, stream)[0])
+    ctr = counters.download(np.uint32, n_lods * 2, stream).reshape(-1, 2)
+    inst = int(ctr[:, 1].sum())
+    out["instanced"] = row(ms, N_OBJECTS * 40 + inst * 4 + n_lods * 8 + k_cmds * 32 + 4,
+                           {"instances": inst, "lod_commands": k_cmds, "instances_per_lod": per_lod,
+                            "largest_list": int(ctr[:, 1].max())})
+    # transform streaming: 1% of the transforms per call (ids + 32-byte transforms from host arrays)
+    n_upd = N_OBJECTS // 100
+    upd = [bz.synth_transform_updates(SEED, f, transforms, n_upd) for f in range(4)]
+    ms, wall = timed(lambda k: ctx.update_transforms(scenes[k % len(scenes)], upd[k % 4][0], upd[k % 4][1], stream))
+    host_bytes = n_upd * (4 + 32)
+    out["transform_updates"] = {"updates_per_call": n_upd, "us": round(ms * 1e3, 2), "wall_us": round(wall * 1e6, 2),
+                                "host_bytes": host_bytes, "pcie_inclusive_gbs": round(host_bytes / wall / 1e9, 2),
+                                "updates_per_s": round(n_upd / wall, 1)}
+    for b in (cmds, cnt, ids, counters, icmds, icnt):
+        b.free()
+    return out
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_frame_c1.json"
     if p.exists():
@@ -262,6 +326,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-clusters", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the PLAIN / ONPC / instanced / transform-update leg")
     ap.add_argument("--served-seconds", type=float, default=0.5)
     ap.add_argument("--no-frame-cache", action="store_true",
                     help="run the C1 passes without BZ_CULL_FRAME_CACHE (the LATE pass reads every object)")
@@ -424,6 +489,10 @@ def main():
     if not args.no_served:
         log("bench: served requests ...")
         result["served"] = bench_served(ctx, (2000, 8000, 32000), args.served_seconds)
+    if not args.no_next:  # last: it rewrites transforms of the resident scene copies
+        log("bench: next rows (plain, onpc, instanced, transform updates) ...")
+        result["next_rows"] = bench_next_rows(ctx, stream, [c[0] for c in copies], v, sc.transforms,
+                                              max(5, min(args.steps, 20)))
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline((sc.objects, sc.transforms, sc.surfaces, sc.lods), bytes(memoryview(v)),

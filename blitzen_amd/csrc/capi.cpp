@@ -898,7 +898,7 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
     if (st_) return st_;
     if (job.n) {
         // chunks of decided tiles: the histogram holds n_chunks x n_lods counters (at most ~16 M)
-        const uint32_t target = std::max<uint32_t>(1u, std::min<uint32_t>(1024u, (1u << 24) / s->n_lods));
+        const uint32_t target = std::max<uint32_t>(1u, std::min<uint32_t>(512u, (1u << 24) / s->n_lods));
         ij.tiles_per_chunk = (job.n_tiles + target - 1) / target;
         ij.n_chunks = (job.n_tiles + ij.tiles_per_chunk - 1) / ij.tiles_per_chunk;
         ij.n_tiles = job.n_tiles;

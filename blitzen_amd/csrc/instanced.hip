@@ -5,12 +5,13 @@
 // of its LOD, and each LOD with instances gets one instanced DrawIndexed command. The reference appends with
 // InterlockedAdd (nondeterministic order, and a full list overflows into the next LOD's); here each list holds its
 // objects in ascending objectId, at most per_lod of them, and the counter keeps the exact count:
-//   hist:    one workgroup per chunk of decided tiles x LOD slice: an LDS histogram of the chunk's LODs;
-//   scan:    one thread per LOD: exclusive prefix of the histogram over chunks (the chunk's first slot in the LOD's
-//            list) and the counter {lod * per_lod, total};
+//   hist:    one workgroup per chunk of decided tiles x LOD slice: an LDS histogram of the chunk's LODs (the
+//            chunk's entries loaded as 64-entry groups, kBatch groups per round trip);
+//   scan:    one workgroup per 64 LODs: exclusive prefix of the histogram over chunks (the chunk's first slot in the
+//            LOD's list) and the counter {lod * per_lod, total};
 //   cmds:    one workgroup: the LODs with instances in ascending order -> 32-byte commands and their number;
-//   scatter: one wave per chunk x LOD slice: the chunk's objects in order, ranked among equal LODs by peeling
-//            (one ballot per distinct LOD in a group of 64), written to lod * per_lod + slot.
+//   scatter: one wave per chunk x LOD slice: the chunk's objects in order, ranked among equal LODs with one ballot
+//            per bit of the LOD index, written to lod * per_lod + slot.
 #include "bz_internal.h"
 
 namespace bz {
@@ -18,53 +19,121 @@ namespace bz {
 namespace {
 
 constexpr uint32_t kSlice = 8192;  // LOD bins per histogram / scatter workgroup (LDS counters)
+constexpr uint32_t kBatch = 8;     // 64-entry groups loaded per round trip
 
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
+// Up to 64 consecutive decided tiles seen as one sequence of 64-entry groups (a tile's last group may be partial):
+// lane k holds tile k's entry count and the inclusive prefix of the group counts.
+struct TileWindow {
+    uint32_t n, gincl, gex, groups;  // per lane (tile t0 + lane); groups = the window's total (wave-uniform)
+};
+
+__device__ __forceinline__ TileWindow load_window(const uint32_t* counts, uint32_t t0, uint32_t nt, uint32_t lane)
+{
+    TileWindow w;
+    w.n = lane < nt ? counts[t0 + lane] : 0u;
+    const uint32_t g = (w.n + 63u) >> 6;
+    w.gincl = wave_incl_scan(g, lane);
+    w.gex = w.gincl - g;
+    w.groups = __shfl(w.gincl, 63, 64);
+    return w;
+}
+
+// Entry `lane` of group g (< w.groups, wave-uniform) of the window: its tile is the first whose inclusive group
+// prefix exceeds g. Returns false for the unused lanes of a partial group.
+__device__ __forceinline__ bool load_group_entry(const TileWindow& w, uint32_t g, const uint2* staging, uint32_t t0,
+                                                 uint32_t tile_size, uint32_t lane, uint2& out)
+{
+    const uint32_t j = (uint32_t)__popcll(__ballot(w.gincl <= g));
+    const uint32_t nj = __shfl(w.n, j, 64), exj = __shfl(w.gex, j, 64);
+    const uint32_t e = (g - exj) * 64u + lane;
+    const bool valid = e < nj;
+    out = valid ? staging[(size_t)(t0 + j) * tile_size + e] : make_uint2(0u, 0u);
+    return valid;
+}
+
+// One workgroup (4 waves) per chunk of decided tiles x LOD slice: an LDS histogram of the chunk's LODs. Each wave
+// takes batches of kBatch groups, all loaded before the first count.
 __global__ __launch_bounds__(256) void inst_hist_kernel(InstancedJob J, const uint32_t* counts, const uint2* staging,
                                                         uint32_t* hist)
 {
     __shared__ uint32_t h[kSlice];
-    const uint32_t c = blockIdx.x, l0 = blockIdx.y * kSlice, tid = threadIdx.x;
+    const uint32_t c = blockIdx.x, l0 = blockIdx.y * kSlice, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t l1 = l0 + kSlice < J.n_lods ? l0 + kSlice : J.n_lods;
     for (uint32_t i = tid; i < l1 - l0; i += 256) h[i] = 0u;
     __syncthreads();
-    const uint32_t t0 = c * J.tiles_per_chunk;
-    const uint32_t t1 = t0 + J.tiles_per_chunk < J.n_tiles ? t0 + J.tiles_per_chunk : J.n_tiles;
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t n = counts[t];
-        const uint2* st = staging + (size_t)t * J.tile_size;
-        for (uint32_t e = tid; e < n; e += 256) {
-            const uint32_t lod = st[e].y;
-            if (lod >= l0 && lod < l1) atomicAdd(&h[lod - l0], 1u);
+    const uint32_t c0 = c * J.tiles_per_chunk;
+    const uint32_t c1 = c0 + J.tiles_per_chunk < J.n_tiles ? c0 + J.tiles_per_chunk : J.n_tiles;
+    for (uint32_t t0 = c0; t0 < c1; t0 += 64) {
+        const uint32_t nt = c1 - t0 < 64u ? c1 - t0 : 64u;
+        const TileWindow w = load_window(counts, t0, nt, lane);
+        for (uint32_t gb = wave * kBatch; gb < w.groups; gb += 4u * kBatch) {
+            uint2 ent[kBatch];
+            bool val[kBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                val[b] = false;
+                if (gb + b < w.groups) val[b] = load_group_entry(w, gb + b, staging, t0, J.tile_size, lane, ent[b]);
+            }
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b)
+                if (val[b] && ent[b].y >= l0 && ent[b].y < l1) atomicAdd(&h[ent[b].y - l0], 1u);
         }
     }
     __syncthreads();
     for (uint32_t i = tid; i < l1 - l0; i += 256) hist[(size_t)c * J.n_lods + l0 + i] = h[i];
 }
 
-// One thread per LOD: the histogram column becomes each chunk's first slot in the LOD's list (in place).
-__global__ __launch_bounds__(256) void inst_scan_kernel(InstancedJob J, uint32_t* hist)
+// One workgroup per 64 LODs (one LOD per lane): wave w holds chunks [w * seg, (w + 1) * seg) of its lane's LOD in
+// registers (every load issued at once), the 16 segment sums are scanned across the waves in LDS, and each chunk's
+// count is replaced by its first slot in the LOD's list; wave 0 writes the counters {lod * per_lod, total}.
+constexpr uint32_t kScanWaves = 16, kScanSeg = 32;  // up to kScanWaves * kScanSeg = 512 chunks
+
+__global__ __launch_bounds__(kScanWaves * 64) void inst_scan_kernel(InstancedJob J, uint32_t* hist)
 {
-    const uint32_t l = blockIdx.x * 256 + threadIdx.x;
-    if (l >= J.n_lods) return;
-    uint32_t run = 0;
-    uint32_t c = 0;
-    for (; c + 8 <= J.n_chunks; c += 8) {  // eight independent loads in flight
-        uint32_t v[8];
+    __shared__ uint32_t seg_sum[kScanWaves][64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t l = blockIdx.x * 64u + lane;
+    const bool ok = l < J.n_lods;
+    const uint32_t segn = (J.n_chunks + kScanWaves - 1u) / kScanWaves;
+    const uint32_t c0 = wave * segn;
+    uint32_t v[kScanSeg];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) v[j] = hist[(size_t)(c + j) * J.n_lods + l];
+    for (uint32_t j = 0; j < kScanSeg; ++j) {
+        const uint32_t c = c0 + j;
+        v[j] = (ok && j < segn && c < J.n_chunks) ? hist[(size_t)c * J.n_lods + l] : 0u;
+    }
+    uint32_t sum = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) {
-            hist[(size_t)(c + j) * J.n_lods + l] = run;
-            run += v[j];
-        }
+    for (uint32_t j = 0; j < kScanSeg; ++j) sum += v[j];
+    seg_sum[wave][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kScanWaves; ++w) {
+        const uint32_t x = seg_sum[w][lane];
+        run += w < wave ? x : 0u;
+        total += x;
     }
-    for (; c < J.n_chunks; ++c) {
-        const uint32_t v = hist[(size_t)c * J.n_lods + l];
-        hist[(size_t)c * J.n_lods + l] = run;
-        run += v;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanSeg; ++j) {
+        const uint32_t c = c0 + j;
+        if (ok && j < segn && c < J.n_chunks) hist[(size_t)c * J.n_lods + l] = run;
+        run += v[j];
     }
-    J.counters[2u * l] = l * J.per_lod;
-    J.counters[2u * l + 1u] = run;
+    if (wave == 0 && ok) {
+        J.counters[2u * l] = l * J.per_lod;
+        J.counters[2u * l + 1u] = total;
+    }
 }
 
 // One workgroup: commands for the LODs with instances, ascending (drawInstCmd.cs.hlsl's fields).
@@ -104,40 +173,50 @@ __global__ __launch_bounds__(1024) void inst_cmd_kernel(InstancedJob J)
 }
 
 // One wave per chunk x LOD slice: the chunk's objects in ascending objectId, each ranked among the chunk's objects of
-// the same LOD seen before it.
+// the same LOD seen before it. The LDS counters start at the chunk's first slot in each LOD's list (the scanned
+// histogram row); within a group of 64 the lanes holding the same LOD are found with one ballot per bit of the LOD's
+// index in the slice (a fixed cost, whatever the number of distinct LODs), and the lowest such lane advances the
+// LOD's counter. Groups are loaded kBatch at a time.
 __global__ __launch_bounds__(64) void inst_scatter_kernel(InstancedJob J, const uint32_t* counts, const uint2* staging,
                                                           const uint32_t* base)
 {
     __shared__ uint32_t cnt[kSlice];
     const uint32_t c = blockIdx.x, l0 = blockIdx.y * kSlice, lane = threadIdx.x;
     const uint32_t l1 = l0 + kSlice < J.n_lods ? l0 + kSlice : J.n_lods;
-    for (uint32_t i = lane; i < l1 - l0; i += 64) cnt[i] = 0u;
+    const uint32_t nbits = l1 - l0 > 1u ? 32u - (uint32_t)__clz(l1 - l0 - 1u) : 0u;  // bits of the slice's largest key
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t i = lane; i < l1 - l0; i += 64) cnt[i] = base[(size_t)c * J.n_lods + l0 + i];
     __syncthreads();
-    const uint32_t t0 = c * J.tiles_per_chunk;
-    const uint32_t t1 = t0 + J.tiles_per_chunk < J.n_tiles ? t0 + J.tiles_per_chunk : J.n_tiles;
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t n = counts[t];
-        const uint2* st = staging + (size_t)t * J.tile_size;
-        for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-            const uint32_t e = e0 + lane;
-            const uint2 en = e < n ? st[e] : make_uint2(0u, 0u);
-            const bool in = e < n && en.y >= l0 && en.y < l1;
-            unsigned long long active = __ballot(in);
-            uint32_t rank = 0;
-            while (active) {
-                const uint32_t leader = (uint32_t)__ffsll((long long)active) - 1u;
-                const uint32_t key = __shfl(en.y, leader, 64);
-                const bool mine = in && en.y == key;
-                const unsigned long long m = __ballot(mine);
-                if (mine) rank = cnt[key - l0] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                __builtin_amdgcn_wave_barrier();
-                if (lane == leader) cnt[key - l0] += (uint32_t)__popcll(m);
-                __builtin_amdgcn_wave_barrier();
-                active &= ~m;
+    const uint32_t c0 = c * J.tiles_per_chunk;
+    const uint32_t c1 = c0 + J.tiles_per_chunk < J.n_tiles ? c0 + J.tiles_per_chunk : J.n_tiles;
+    for (uint32_t t0 = c0; t0 < c1; t0 += 64) {
+        const uint32_t nt = c1 - t0 < 64u ? c1 - t0 : 64u;
+        const TileWindow w = load_window(counts, t0, nt, lane);
+        for (uint32_t gb = 0; gb < w.groups; gb += kBatch) {
+            uint2 ent[kBatch];
+            bool val[kBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                val[b] = false;
+                if (gb + b < w.groups) val[b] = load_group_entry(w, gb + b, staging, t0, J.tile_size, lane, ent[b]);
             }
-            if (in) {
-                const uint32_t pos = base[(size_t)c * J.n_lods + en.y] + rank;
-                if (pos < J.per_lod) J.instance_ids[(size_t)en.y * J.per_lod + pos] = en.x;
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                if (gb + b >= w.groups) break;
+                const bool in = val[b] && ent[b].y >= l0 && ent[b].y < l1;
+                const uint32_t key = in ? ent[b].y - l0 : 0u;
+                unsigned long long eq = __ballot(in);  // lanes in the slice with this lane's key
+                for (uint32_t bit = 0; bit < nbits; ++bit) {
+                    const bool set = (key >> bit) & 1u;
+                    const unsigned long long m = __ballot(set);
+                    eq &= set ? m : ~m;
+                }
+                const uint32_t rank = (uint32_t)__popcll(eq & below);
+                const uint32_t pos = in ? cnt[key] + rank : 0u;  // every lane reads before any leader writes
+                __builtin_amdgcn_wave_barrier();
+                if (in && rank == 0u) cnt[key] = pos + (uint32_t)__popcll(eq);
+                __builtin_amdgcn_wave_barrier();
+                if (in && pos < J.per_lod) J.instance_ids[(size_t)ent[b].y * J.per_lod + pos] = ent[b].x;
             }
         }
     }
@@ -148,13 +227,14 @@ __global__ __launch_bounds__(64) void inst_scatter_kernel(InstancedJob J, const 
 hipError_t launch_instanced(const InstancedJob& J, const CompactScratch& s, uint32_t* hist, hipStream_t st)
 {
     const uint32_t slices = (J.n_lods + kSlice - 1) / kSlice;
+    if (J.n_chunks > kScanWaves * kScanSeg) return hipErrorInvalidValue;  // the scan holds at most 512 chunks
     if (J.n_chunks) {
         hipLaunchKernelGGL(inst_hist_kernel, dim3(J.n_chunks, slices), dim3(256), 0, st, J,
                            (const uint32_t*)s.counts, (const uint2*)s.staging, hist);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(inst_scan_kernel, dim3((J.n_lods + 255) / 256), dim3(256), 0, st, J, hist);
+    hipLaunchKernelGGL(inst_scan_kernel, dim3((J.n_lods + 63) / 64), dim3(kScanWaves * 64), 0, st, J, hist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(inst_cmd_kernel, dim3(1), dim3(1024), 0, st, J);

@@ -76,3 +76,21 @@ def test_instanced_tunings_and_shared_transforms(ctx):
     finally:
         ctx.set_tuning(0, old.block, old.items, old.grid_per_cu)
     ctx.scene_destroy(h)
+
+
+def test_instanced_more_than_64_tiles_per_chunk(ctx):
+    """A chunk of decided tiles is read in windows of 64 tiles: with 512-object tiles, 17 039 360 objects make
+    33 280 tiles, 65 per chunk (the chunk count is capped at 512), so every chunk spans two windows."""
+    n = 512 * 512 * 65
+    sc = bz.synth_scene(H.SEED_C0 + 7, n, 64, 4)
+    scene = H.as_tuple(sc)
+    h = ctx.scene_create(*scene)
+    v = H.view()
+    v.lodTarget = 1e-4
+    old = ctx.get_tuning(0)
+    try:
+        ctx.set_tuning(0, 256, 2)
+        check(run(ctx, h, scene, v, 64), orc.cull_instanced(scene, H.view_bytes(v), 64))
+    finally:
+        ctx.set_tuning(0, old.block, old.items, old.grid_per_cu)
+    ctx.scene_destroy(h)
