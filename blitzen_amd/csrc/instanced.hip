@@ -10,8 +10,9 @@
 //   scan:    one workgroup per 64 LODs: exclusive prefix of the histogram over chunks (the chunk's first slot in the
 //            LOD's list) and the counter {lod * per_lod, total};
 //   cmds:    one workgroup: the LODs with instances in ascending order -> 32-byte commands and their number;
-//   scatter: one wave per chunk x LOD slice: the chunk's objects in order, ranked among equal LODs with one ballot
-//            per bit of the LOD index, written to lod * per_lod + slot.
+//   scatter: one workgroup per chunk x LOD slice: per-wave LOD counts over quarters of the chunk's entries, a hand-over
+//            of each LOD's running slot through the quarters, then ranks among equal LODs with one ballot per bit of
+//            the LOD index; each object written to lod * per_lod + slot.
 #include "bz_internal.h"
 
 namespace bz {
@@ -172,37 +173,91 @@ __global__ __launch_bounds__(1024) void inst_cmd_kernel(InstancedJob J)
     if (tid == 0) *J.count = s_base;
 }
 
-// One wave per chunk x LOD slice: the chunk's objects in ascending objectId, each ranked among the chunk's objects of
-// the same LOD seen before it. The LDS counters start at the chunk's first slot in each LOD's list (the scanned
-// histogram row); within a group of 64 the lanes holding the same LOD are found with one ballot per bit of the LOD's
-// index in the slice (a fixed cost, whatever the number of distinct LODs), and the lowest such lane advances the
-// LOD's counter. Groups are loaded kBatch at a time.
-__global__ __launch_bounds__(64) void inst_scatter_kernel(InstancedJob J, const uint32_t* counts, const uint2* staging,
-                                                          const uint32_t* base)
+// One workgroup (4 waves) per chunk x LOD slice of kScatterSlice LODs. The chunk's entries form one sequence of
+// 64-entry groups (ascending objectId) per window of up to 64 tiles; wave w takes the w-th quarter of the groups.
+//   1. each wave counts its quarter's LODs into its own LDS histogram;
+//   2. per LOD, the running slot (starting at the chunk's first slot in the LOD's list, from the scanned histogram)
+//      is handed through the four quarters in order: each wave's histogram becomes its first slot per LOD;
+//   3. each wave ranks its entries among equal LODs in its quarter, in order: the lanes holding the same LOD are found
+//      with one ballot per bit of the LOD's index in the slice, and the lowest such lane advances the LOD's slot.
+constexpr uint32_t kScatterSlice = 2048;
+constexpr uint32_t kScatterWaves = 4;
+
+__global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(InstancedJob J, const uint32_t* counts,
+                                                                          const uint2* staging, const uint32_t* base)
 {
-    __shared__ uint32_t cnt[kSlice];
-    const uint32_t c = blockIdx.x, l0 = blockIdx.y * kSlice, lane = threadIdx.x;
-    const uint32_t l1 = l0 + kSlice < J.n_lods ? l0 + kSlice : J.n_lods;
-    const uint32_t nbits = l1 - l0 > 1u ? 32u - (uint32_t)__clz(l1 - l0 - 1u) : 0u;  // bits of the slice's largest key
+    __shared__ uint32_t slot[kScatterWaves][kScatterSlice];  // per wave: LOD -> count (1), then next slot (2, 3)
+    __shared__ uint32_t running[kScatterSlice];              // per LOD: the next slot after the windows so far
+    const uint32_t c = blockIdx.x, l0 = blockIdx.y * kScatterSlice, tid = threadIdx.x, lane = tid & 63u,
+                   wave = tid >> 6;
+    const uint32_t l1 = l0 + kScatterSlice < J.n_lods ? l0 + kScatterSlice : J.n_lods;
+    const uint32_t nl = l1 - l0;
+    const uint32_t nbits = nl > 1u ? 32u - (uint32_t)__clz(nl - 1u) : 0u;  // bits of the slice's largest key
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (uint32_t i = lane; i < l1 - l0; i += 64) cnt[i] = base[(size_t)c * J.n_lods + l0 + i];
-    __syncthreads();
+    {
+        const uint32_t* row = base + (size_t)c * J.n_lods + l0;
+        uint32_t v[kScatterSlice / (kScatterWaves * 64)];
+#pragma unroll
+        for (uint32_t k = 0; k < kScatterSlice / (kScatterWaves * 64); ++k) {
+            const uint32_t i = k * kScatterWaves * 64u + tid;
+            v[k] = i < nl ? row[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kScatterSlice / (kScatterWaves * 64); ++k) {
+            const uint32_t i = k * kScatterWaves * 64u + tid;
+            if (i < nl) running[i] = v[k];
+        }
+    }
     const uint32_t c0 = c * J.tiles_per_chunk;
     const uint32_t c1 = c0 + J.tiles_per_chunk < J.n_tiles ? c0 + J.tiles_per_chunk : J.n_tiles;
     for (uint32_t t0 = c0; t0 < c1; t0 += 64) {
         const uint32_t nt = c1 - t0 < 64u ? c1 - t0 : 64u;
         const TileWindow w = load_window(counts, t0, nt, lane);
-        for (uint32_t gb = 0; gb < w.groups; gb += kBatch) {
-            uint2 ent[kBatch];
-            bool val[kBatch];
+        const uint32_t q = (w.groups + kScatterWaves - 1u) / kScatterWaves;  // groups per wave
+        const uint32_t g0 = wave * q < w.groups ? wave * q : w.groups;
+        const uint32_t g1 = g0 + q < w.groups ? g0 + q : w.groups;
+        for (uint32_t i = lane; i < nl; i += 64) slot[wave][i] = 0u;
+        __syncthreads();  // (also orders `running` from the previous window or the seeding)
+        // 1. this wave's LOD counts
+        uint2 ent[kBatch];
+        bool val[kBatch];
+        for (uint32_t gb = g0; gb < g1; gb += kBatch) {
 #pragma unroll
             for (uint32_t b = 0; b < kBatch; ++b) {
                 val[b] = false;
-                if (gb + b < w.groups) val[b] = load_group_entry(w, gb + b, staging, t0, J.tile_size, lane, ent[b]);
+                if (gb + b < g1) val[b] = load_group_entry(w, gb + b, staging, t0, J.tile_size, lane, ent[b]);
+            }
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b)
+                if (val[b] && ent[b].y >= l0 && ent[b].y < l1) atomicAdd(&slot[wave][ent[b].y - l0], 1u);
+        }
+        __syncthreads();
+        // 2. hand the running slot through the waves' quarters, per LOD
+        for (uint32_t i = tid; i < nl; i += kScatterWaves * 64u) {
+            uint32_t run = running[i];
+#pragma unroll
+            for (uint32_t v = 0; v < kScatterWaves; ++v) {
+                const uint32_t n = slot[v][i];
+                slot[v][i] = run;
+                run += n;
+            }
+            running[i] = run;
+        }
+        __syncthreads();
+        // 3. ranks in order within this wave's quarter (the entries of the last batch are still in registers when the
+        // quarter fits one batch)
+        const bool reuse = g1 - g0 <= kBatch;
+        for (uint32_t gb = g0; gb < g1; gb += kBatch) {
+            if (!reuse) {
+#pragma unroll
+                for (uint32_t b = 0; b < kBatch; ++b) {
+                    val[b] = false;
+                    if (gb + b < g1) val[b] = load_group_entry(w, gb + b, staging, t0, J.tile_size, lane, ent[b]);
+                }
             }
 #pragma unroll
             for (uint32_t b = 0; b < kBatch; ++b) {
-                if (gb + b >= w.groups) break;
+                if (gb + b >= g1) break;
                 const bool in = val[b] && ent[b].y >= l0 && ent[b].y < l1;
                 const uint32_t key = in ? ent[b].y - l0 : 0u;
                 unsigned long long eq = __ballot(in);  // lanes in the slice with this lane's key
@@ -212,13 +267,14 @@ __global__ __launch_bounds__(64) void inst_scatter_kernel(InstancedJob J, const 
                     eq &= set ? m : ~m;
                 }
                 const uint32_t rank = (uint32_t)__popcll(eq & below);
-                const uint32_t pos = in ? cnt[key] + rank : 0u;  // every lane reads before any leader writes
+                const uint32_t pos = in ? slot[wave][key] + rank : 0u;  // every lane reads before any leader writes
                 __builtin_amdgcn_wave_barrier();
-                if (in && rank == 0u) cnt[key] = pos + (uint32_t)__popcll(eq);
+                if (in && rank == 0u) slot[wave][key] = pos + (uint32_t)__popcll(eq);
                 __builtin_amdgcn_wave_barrier();
                 if (in && pos < J.per_lod) J.instance_ids[(size_t)ent[b].y * J.per_lod + pos] = ent[b].x;
             }
         }
+        __syncthreads();  // slot[] is cleared by the next window
     }
 }
 
@@ -240,8 +296,9 @@ hipError_t launch_instanced(const InstancedJob& J, const CompactScratch& s, uint
     hipLaunchKernelGGL(inst_cmd_kernel, dim3(1), dim3(1024), 0, st, J);
     e = hipGetLastError();
     if (e != hipSuccess || !J.n_chunks) return e;
-    hipLaunchKernelGGL(inst_scatter_kernel, dim3(J.n_chunks, slices), dim3(64), 0, st, J, (const uint32_t*)s.counts,
-                       (const uint2*)s.staging, (const uint32_t*)hist);
+    const uint32_t scatter_slices = (J.n_lods + kScatterSlice - 1) / kScatterSlice;
+    hipLaunchKernelGGL(inst_scatter_kernel, dim3(J.n_chunks, scatter_slices), dim3(kScatterWaves * 64), 0, st, J,
+                       (const uint32_t*)s.counts, (const uint2*)s.staging, (const uint32_t*)hist);
     return hipGetLastError();
 }
 
