@@ -300,6 +300,24 @@ def bench_next_rows(ctx, stream, scenes, view, transforms, steps: int):
     out["transform_updates"] = {"updates_per_call": n_upd, "us": round(ms * 1e3, 2), "wall_us": round(wall * 1e6, 2),
                                 "host_bytes": host_bytes, "pcie_inclusive_gbs": round(host_bytes / wall / 1e9, 2),
                                 "updates_per_s": round(n_upd / wall, 1)}
+    # the same updates written by the caller straight into the pinned staging slots (filled once per slot before the
+    # timed calls; each call hands out the next slot and commits it: two uploads and the device scatter)
+    h0 = scenes[0]
+    for k in range(2):
+        sid, sxf = ctx.transform_staging(h0, n_upd)
+        sid[:] = upd[k][0]
+        sxf[:] = upd[k][1]
+        ctx.commit_transforms(h0, n_upd, stream)
+
+    def staged(k):
+        ctx.transform_staging(h0, n_upd)
+        ctx.commit_transforms(h0, n_upd, stream)
+
+    ms, wall = timed(staged)
+    out["transform_updates_staged"] = {"updates_per_call": n_upd, "us": round(ms * 1e3, 2),
+                                       "wall_us": round(wall * 1e6, 2), "host_bytes": host_bytes,
+                                       "pcie_inclusive_gbs": round(host_bytes / wall / 1e9, 2),
+                                       "updates_per_s": round(n_upd / wall, 1)}
     for b in (cmds, cnt, ids, counters, icmds, icnt):
         b.free()
     return out

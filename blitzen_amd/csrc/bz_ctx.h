@@ -52,6 +52,16 @@ struct Scene {
     void* upd_dev = nullptr;
     size_t upd_cap = 0;
     hipEvent_t upd_event = nullptr;
+    // caller-filled staging (bz_scene_transform_staging): two pinned slots [ids | 32-byte MeshTransforms], used in turn
+    struct XfSlot {
+        uint8_t* host = nullptr;       // pinned: cap ids, then cap transforms at xf_offset
+        uint8_t* dev = nullptr;        // device copy of the same layout
+        uint32_t cap = 0;
+        size_t xf_offset = 0;
+        hipEvent_t done = nullptr;     // the last commit from this slot completed
+    } xf_slot[2];
+    uint32_t xf_next = 0;              // slot handed out by the next staging call
+    int xf_open = -1;                  // slot handed out and not yet committed
     SceneDev dev() const;
     ~Scene();
 };
@@ -82,7 +92,7 @@ struct Pyramid : PyrPlan {
 
 struct StreamScratch {
     CompactScratch cs{};
-    uint32_t* err = nullptr;          // [0] spare, [1] pyramid tail wait gave up, [2..3] spare
+    uint32_t* err = nullptr;          // [0] spare, [1] pyramid tail wait gave up, [2] staged transform id out of range
     CullJob* jobs_dev = nullptr;
     CullJob* jobs_host = nullptr;     // pinned
     uint32_t jobs_cap = 0;

@@ -112,6 +112,10 @@ hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, floa
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t st);
 hipError_t launch_scatter_transforms(const uint32_t* ids, const float4* src_ps, const float4* src_q, uint32_t n,
                                      float4* xf_ps, float4* xf_q, hipStream_t st);
+// Staged updates: ids[i] and the 32-byte MeshTransform record i (pos.xyz, scale | quaternion) scattered into the SoA
+// streams; an id >= n_xf is skipped and sets err[2].
+hipError_t launch_scatter_staged_transforms(const uint32_t* ids, const float4* records, uint32_t n, uint32_t n_xf,
+                                            float4* xf_ps, float4* xf_q, uint32_t* err, hipStream_t st);
 hipError_t launch_synth_depth(uint64_t seed, uint32_t w, uint32_t h, uint32_t occluders, float p11, float znear,
                               float* depth, hipStream_t st);
 

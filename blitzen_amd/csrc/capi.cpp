@@ -74,6 +74,12 @@ Scene::~Scene()
         if (p) (void)hipFree(p);
     if (upd_host) (void)hipHostFree(upd_host);
     if (upd_event) (void)hipEventDestroy(upd_event);
+    for (XfSlot& x : xf_slot) {
+        if (x.done) (void)hipEventSynchronize(x.done);
+        if (x.host) (void)hipHostFree(x.host);
+        if (x.dev) (void)hipFree(x.dev);
+        if (x.done) (void)hipEventDestroy(x.done);
+    }
 }
 
 Pyramid::~Pyramid()
@@ -514,6 +520,72 @@ bz_status bz_scene_info(bz_ctx* ctx, bz_scene scene, uint32_t* n_objects, uint32
     if (n_surfaces) *n_surfaces = s->n_surfaces;
     if (n_lods) *n_lods = s->n_lods;
     if (device_bytes) *device_bytes = s->bytes;
+    return BZ_OK;
+}
+
+bz_status bz_scene_transform_staging(bz_ctx* ctx, bz_scene scene, uint32_t capacity, uint32_t** ids,
+                                     bz_MeshTransform** transforms)
+{
+    if (!ctx || !ids || !transforms) return BZ_E_INVALID;
+    *ids = nullptr;
+    *transforms = nullptr;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (!capacity) return ctx->fail(BZ_E_INVALID, "capacity must be > 0");
+    const uint32_t k = s->xf_open >= 0 ? (uint32_t)s->xf_open : s->xf_next;  // an open slot is handed out again
+    Scene::XfSlot& x = s->xf_slot[k];
+    if (x.done) {
+        hipError_t e = hipEventSynchronize(x.done);  // the slot's previous upload has been read
+        if (e != hipSuccess) return ctx->hip_fail(e, "staging slot wait");
+    }
+    if (x.cap < capacity) {
+        if (x.host) (void)hipHostFree(x.host);
+        if (x.dev) (void)hipFree(x.dev);
+        x.host = x.dev = nullptr;
+        x.cap = 0;
+        const size_t off = ((size_t)capacity * 4 + 255) / 256 * 256;
+        const size_t bytes = off + (size_t)capacity * sizeof(bz_MeshTransform);
+        hipError_t e = hipHostMalloc((void**)&x.host, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&x.dev, bytes);
+        if (e == hipSuccess && !x.done) e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
+        if (e != hipSuccess) return ctx->hip_fail(e, "transform staging slot");
+        x.cap = capacity;
+        x.xf_offset = off;
+    }
+    s->xf_open = (int)k;
+    *ids = reinterpret_cast<uint32_t*>(x.host);
+    *transforms = reinterpret_cast<bz_MeshTransform*>(x.host + x.xf_offset);
+    return BZ_OK;
+}
+
+bz_status bz_scene_commit_transforms(bz_ctx* ctx, bz_scene scene, uint32_t n, void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (s->xf_open < 0) return ctx->fail(BZ_E_STATE, "no staging slot handed out (bz_scene_transform_staging)");
+    Scene::XfSlot& x = s->xf_slot[s->xf_open];
+    if (n > x.cap) return ctx->fail(BZ_E_INVALID, "more updates than the staging slot holds");
+    hipStream_t st = (hipStream_t)stream;
+    StreamScratch* sc = nullptr;
+    bz_status status = ctx->scratch_for(st, 0, 0, &sc);  // the stream's error flags (no compaction scratch)
+    if (status) return status;
+    ++s->xf_version;  // frame-cache survivors recorded before this update no longer apply
+    hipError_t e = hipSuccess;
+    if (n) {
+        e = hipMemcpyAsync(x.dev, x.host, (size_t)n * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(x.dev + x.xf_offset, x.host + x.xf_offset, (size_t)n * sizeof(bz_MeshTransform),
+                               hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = launch_scatter_staged_transforms(reinterpret_cast<const uint32_t*>(x.dev),
+                                                 reinterpret_cast<const float4*>(x.dev + x.xf_offset), n,
+                                                 s->n_transforms, s->xf_ps, s->xf_q, sc->err, st);
+    }
+    if (e == hipSuccess) e = hipEventRecord(x.done, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "staged transform update");
+    s->xf_next = (uint32_t)s->xf_open ^ 1u;
+    s->xf_open = -1;
     return BZ_OK;
 }
 
@@ -1098,8 +1170,10 @@ bz_status bz_check(bz_ctx* ctx, void* stream)
     uint32_t flags[4] = { 0, 0, 0, 0 };
     e = hipMemcpy(flags, it->second->err, sizeof flags, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return ctx->hip_fail(e, "error flag read");
-    if (flags[0] || flags[1]) {
+    if (flags[0] || flags[1] || flags[2]) {
         (void)hipMemset(it->second->err, 0, 16);
+        if (flags[2] && !flags[0] && !flags[1])
+            return ctx->fail(BZ_E_INVALID, "a staged transform update named a transform id out of range (skipped)");
         return ctx->fail(BZ_E_INTERNAL, "a kernel gave up a bounded wait");
     }
     return BZ_OK;

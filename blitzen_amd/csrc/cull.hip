@@ -1007,4 +1007,28 @@ hipError_t launch_scatter_transforms(const uint32_t* ids, const float4* src_ps, 
     return hipGetLastError();
 }
 
+__global__ void scatter_staged_transforms_kernel(const uint32_t* __restrict__ ids, const float4* __restrict__ records,
+                                                 uint32_t n, uint32_t n_xf, float4* xf_ps, float4* xf_q, uint32_t* err)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = ids[i];
+    const float4 ps = records[2u * i], q = records[2u * i + 1u];
+    if (id >= n_xf) {
+        atomicOr(&err[2], 1u);
+        return;
+    }
+    xf_ps[id] = ps;
+    xf_q[id] = q;
+}
+
+hipError_t launch_scatter_staged_transforms(const uint32_t* ids, const float4* records, uint32_t n, uint32_t n_xf,
+                                            float4* xf_ps, float4* xf_q, uint32_t* err, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_staged_transforms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ids, records, n, n_xf,
+                       xf_ps, xf_q, err);
+    return hipGetLastError();
+}
+
 }  // namespace bz

@@ -169,6 +169,20 @@ class Context:
         self.check(lib().bz_scene_update_transforms(self.handle, scene, A.ptr(ids), A.ptr(transforms), len(ids),
                                                     stream), "bz_scene_update_transforms")
 
+    def transform_staging(self, scene: int, capacity: int):
+        """Pinned staging slot as numpy views (ids uint32[capacity], transforms MESH_TRANSFORM[capacity]); fill the
+        first n entries, then commit_transforms(scene, n)."""
+        ids_p, xf_p = C.c_void_p(), C.c_void_p()
+        self.check(lib().bz_scene_transform_staging(self.handle, scene, capacity, C.byref(ids_p), C.byref(xf_p)),
+                   "bz_scene_transform_staging")
+        ids = np.ctypeslib.as_array((C.c_uint32 * capacity).from_address(ids_p.value))
+        xf = np.frombuffer((C.c_uint8 * (capacity * MESH_TRANSFORM.itemsize)).from_address(xf_p.value),
+                           dtype=MESH_TRANSFORM)
+        return ids, xf
+
+    def commit_transforms(self, scene: int, n: int, stream=None):
+        self.check(lib().bz_scene_commit_transforms(self.handle, scene, n, stream), "bz_scene_commit_transforms")
+
     # ------------------------------------------------------------------ pyramids
     def pyramid_create(self, base_w: int, base_h: int, mips: int) -> int:
         h = C.c_uint64()

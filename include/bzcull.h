@@ -175,6 +175,16 @@ BZ_API bz_status bz_scene_destroy(bz_ctx* ctx, bz_scene scene);
  * (host arrays) into the resident transform streams, ordered on `stream`. Ids must be < n_transforms. */
 BZ_API bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const uint32_t* transform_ids,
                                      const bz_MeshTransform* transforms, uint32_t n, void* stream);
+/* Caller-filled staging for transform updates, as the reference's persistently mapped staging buffer
+ * (vulkanRendererSetup.cpp:298-302) that UpdateObjectTransform writes into (vulkanDraw.cpp:816-820): returns pinned
+ * host arrays of `capacity` transform ids and MeshTransforms (32 B records, the reference layout). The caller fills the
+ * first n entries and calls bz_scene_commit_transforms, which uploads them and scatters them on `stream` (no host copy
+ * or conversion). Two slots are used in turn: the caller fills one while the other's upload runs; handing a slot out
+ * again waits for its previous upload. Calling the staging function again before committing returns the same slot.
+ * Ids are checked on the device: an id >= n_transforms is skipped and bz_check(stream) reports BZ_E_INVALID. */
+BZ_API bz_status bz_scene_transform_staging(bz_ctx* ctx, bz_scene scene, uint32_t capacity, uint32_t** ids,
+                                            bz_MeshTransform** transforms);
+BZ_API bz_status bz_scene_commit_transforms(bz_ctx* ctx, bz_scene scene, uint32_t n, void* stream);
 /* Overwrites the last-frame visibility buffer from a host array of n_objects uint32 (0 / non-zero). */
 BZ_API bz_status bz_scene_set_visibility(bz_ctx* ctx, bz_scene scene, const uint32_t* visibility, uint32_t n, void* stream);
 /* Copies the visibility buffer to a host array (synchronizes `stream`). */
