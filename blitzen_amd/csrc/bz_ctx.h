@@ -47,15 +47,10 @@ struct Scene {
         uint64_t xf_version = 0;
         const uint32_t* vis_in = nullptr;  // the last-frame visibility the EARLY pass read (recorded per survivor)
     } cache;
-    // transform-update staging
-    void* upd_host = nullptr;
-    void* upd_dev = nullptr;
-    size_t upd_cap = 0;
-    hipEvent_t upd_event = nullptr;
-    // caller-filled staging (bz_scene_transform_staging): two pinned slots [ids | 32-byte MeshTransforms], used in turn
+    // transform-update staging (bz_scene_transform_staging, also behind bz_scene_update_transforms): two pinned slots
+    // [ids | 32-byte MeshTransforms], used in turn
     struct XfSlot {
-        uint8_t* host = nullptr;       // pinned: cap ids, then cap transforms at xf_offset
-        uint8_t* dev = nullptr;        // device copy of the same layout
+        uint8_t* host = nullptr;       // pinned, read by the scatter kernel: cap ids, then cap transforms at xf_offset
         uint32_t cap = 0;
         size_t xf_offset = 0;
         hipEvent_t done = nullptr;     // the last commit from this slot completed

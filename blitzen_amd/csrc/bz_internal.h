@@ -110,8 +110,6 @@ hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, const uint32_t* sta
 hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
                                 hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t st);
-hipError_t launch_scatter_transforms(const uint32_t* ids, const float4* src_ps, const float4* src_q, uint32_t n,
-                                     float4* xf_ps, float4* xf_q, hipStream_t st);
 // Staged updates: ids[i] and the 32-byte MeshTransform record i (pos.xyz, scale | quaternion) scattered into the SoA
 // streams; an id >= n_xf is skipped and sets err[2].
 hipError_t launch_scatter_staged_transforms(const uint32_t* ids, const float4* records, uint32_t n, uint32_t n_xf,

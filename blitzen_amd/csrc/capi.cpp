@@ -69,15 +69,12 @@ Pyramid* Ctx::pyramid(bz_pyramid h)
 Scene::~Scene()
 {
     void* ps[] = { obj_surf, obj_xf, xf_ps, xf_q, surf_sphere, surf_lod, lod_error, lod_draw, lod_cluster, vis,
-                   cl_sphere, cl_cone, cl_draw, cache.sph, cache.meta, cache.counts, upd_dev };
+                   cl_sphere, cl_cone, cl_draw, cache.sph, cache.meta, cache.counts };
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    if (upd_host) (void)hipHostFree(upd_host);
-    if (upd_event) (void)hipEventDestroy(upd_event);
     for (XfSlot& x : xf_slot) {
         if (x.done) (void)hipEventSynchronize(x.done);
         if (x.host) (void)hipHostFree(x.host);
-        if (x.dev) (void)hipFree(x.dev);
         if (x.done) (void)hipEventDestroy(x.done);
     }
 }
@@ -540,13 +537,11 @@ bz_status bz_scene_transform_staging(bz_ctx* ctx, bz_scene scene, uint32_t capac
     }
     if (x.cap < capacity) {
         if (x.host) (void)hipHostFree(x.host);
-        if (x.dev) (void)hipFree(x.dev);
-        x.host = x.dev = nullptr;
+        x.host = nullptr;
         x.cap = 0;
         const size_t off = ((size_t)capacity * 4 + 255) / 256 * 256;
         const size_t bytes = off + (size_t)capacity * sizeof(bz_MeshTransform);
-        hipError_t e = hipHostMalloc((void**)&x.host, bytes, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipMalloc((void**)&x.dev, bytes);
+        hipError_t e = hipHostMalloc((void**)&x.host, bytes, hipHostMallocDefault);  // pinned, device-readable
         if (e == hipSuccess && !x.done) e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
         if (e != hipSuccess) return ctx->hip_fail(e, "transform staging slot");
         x.cap = capacity;
@@ -573,14 +568,10 @@ bz_status bz_scene_commit_transforms(bz_ctx* ctx, bz_scene scene, uint32_t n, vo
     ++s->xf_version;  // frame-cache survivors recorded before this update no longer apply
     hipError_t e = hipSuccess;
     if (n) {
-        e = hipMemcpyAsync(x.dev, x.host, (size_t)n * 4, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(x.dev + x.xf_offset, x.host + x.xf_offset, (size_t)n * sizeof(bz_MeshTransform),
-                               hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = launch_scatter_staged_transforms(reinterpret_cast<const uint32_t*>(x.dev),
-                                                 reinterpret_cast<const float4*>(x.dev + x.xf_offset), n,
-                                                 s->n_transforms, s->xf_ps, s->xf_q, sc->err, st);
+        // the scatter kernel reads the pinned slot directly over the host link (one pass, no staging copy)
+        e = launch_scatter_staged_transforms(reinterpret_cast<const uint32_t*>(x.host),
+                                             reinterpret_cast<const float4*>(x.host + x.xf_offset), n,
+                                             s->n_transforms, s->xf_ps, s->xf_q, sc->err, st);
     }
     if (e == hipSuccess) e = hipEventRecord(x.done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "staged transform update");
@@ -599,57 +590,15 @@ bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const uint32_t
     if (!ids || !xf) return ctx->fail(BZ_E_INVALID, "NULL ids / transforms");
     for (uint32_t k = 0; k < n; ++k)
         if (ids[k] >= s->n_transforms) return ctx->fail(BZ_E_INVALID, "transform id out of range");
-    ++s->xf_version;  // frame-cache survivors recorded before this update no longer apply
-    hipStream_t st = (hipStream_t)stream;
-    // staging: one pinned block [ids | ps | q], reused once the previous upload on this scene completed
-    const size_t need = (size_t)n * (4 + 32);
-    if (s->upd_cap < need) {
-        if (s->upd_event) (void)hipEventSynchronize(s->upd_event);
-        if (s->upd_host) (void)hipHostFree(s->upd_host);
-        if (s->upd_dev) (void)hipFree(s->upd_dev);
-        s->upd_host = nullptr;
-        s->upd_dev = nullptr;
-        s->upd_cap = 0;
-        hipError_t e = hipHostMalloc((void**)&s->upd_host, need, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipMalloc((void**)&s->upd_dev, need);
-        if (e == hipSuccess && !s->upd_event) e = hipEventCreateWithFlags(&s->upd_event, hipEventDisableTiming);
-        if (e != hipSuccess) return ctx->hip_fail(e, "transform staging");
-        s->upd_cap = need;
-    } else if (s->upd_event) {
-        (void)hipEventSynchronize(s->upd_event);
-    }
-    uint8_t* h = (uint8_t*)s->upd_host;
-    std::memcpy(h, ids, (size_t)n * 4);
-    const size_t ps_off = ((size_t)n * 4 + 15) / 16 * 16;
-    float4* hps = (float4*)(h + ps_off);
-    float4* hq = hps + n;
-    if (ps_off + (size_t)n * 32 > s->upd_cap) {
-        // alignment padding needs a little more room
-        (void)hipEventSynchronize(s->upd_event);
-        (void)hipHostFree(s->upd_host);
-        (void)hipFree(s->upd_dev);
-        const size_t need2 = ps_off + (size_t)n * 32;
-        hipError_t e = hipHostMalloc((void**)&s->upd_host, need2, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipMalloc((void**)&s->upd_dev, need2);
-        if (e != hipSuccess) return ctx->hip_fail(e, "transform staging");
-        s->upd_cap = need2;
-        h = (uint8_t*)s->upd_host;
-        std::memcpy(h, ids, (size_t)n * 4);
-        hps = (float4*)(h + ps_off);
-        hq = hps + n;
-    }
-    for (uint32_t k = 0; k < n; ++k) {
-        hps[k] = make_float4(xf[k].pos[0], xf[k].pos[1], xf[k].pos[2], xf[k].scale);
-        hq[k] = make_float4(xf[k].orientation[0], xf[k].orientation[1], xf[k].orientation[2], xf[k].orientation[3]);
-    }
-    uint8_t* d = (uint8_t*)s->upd_dev;
-    hipError_t e = hipMemcpyAsync(d, h, ps_off + (size_t)n * 32, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = launch_scatter_transforms((const uint32_t*)d, (const float4*)(d + ps_off), (const float4*)(d + ps_off) + n, n,
-                                      s->xf_ps, s->xf_q, st);
-    if (e == hipSuccess) e = hipEventRecord(s->upd_event, st);
-    if (e != hipSuccess) return ctx->hip_fail(e, "transform update");
-    return BZ_OK;
+    if (s->xf_open >= 0) return ctx->fail(BZ_E_STATE, "a transform staging slot is open: commit it first");
+    // through the staging slots: the host arrays are copied as they are (32-byte records), the kernel splits them
+    uint32_t* sid = nullptr;
+    bz_MeshTransform* sxf = nullptr;
+    bz_status st = bz_scene_transform_staging(ctx, scene, n, &sid, &sxf);
+    if (st) return st;
+    std::memcpy(sid, ids, (size_t)n * 4);
+    std::memcpy(sxf, xf, (size_t)n * sizeof(bz_MeshTransform));
+    return bz_scene_commit_transforms(ctx, scene, n, stream);
 }
 
 bz_status bz_scene_set_visibility(bz_ctx* ctx, bz_scene scene, const uint32_t* vis, uint32_t n, void* stream)

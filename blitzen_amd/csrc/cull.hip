@@ -988,25 +988,6 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
     return hipGetLastError();
 }
 
-__global__ void scatter_transforms_kernel(const uint32_t* __restrict__ ids, const float4* __restrict__ src_ps,
-                                          const float4* __restrict__ src_q, uint32_t n, float4* xf_ps, float4* xf_q)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t id = ids[i];
-    xf_ps[id] = src_ps[i];
-    xf_q[id] = src_q[i];
-}
-
-hipError_t launch_scatter_transforms(const uint32_t* ids, const float4* src_ps, const float4* src_q, uint32_t n,
-                                     float4* xf_ps, float4* xf_q, hipStream_t st)
-{
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(scatter_transforms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ids, src_ps, src_q, n, xf_ps,
-                       xf_q);
-    return hipGetLastError();
-}
-
 __global__ void scatter_staged_transforms_kernel(const uint32_t* __restrict__ ids, const float4* __restrict__ records,
                                                  uint32_t n, uint32_t n_xf, float4* xf_ps, float4* xf_q, uint32_t* err)
 {
