@@ -160,8 +160,8 @@ def bench_served(ctx, rates, duration_s: float):
     log(f"bench: served scenes ({sum(n for _, n in scenes)} objects) resident in {time.perf_counter() - t0:.1f}s")
     budget = 64 << 30
     ctx.server_start(budget, inflight=4, max_batch=64, flags=A.SERVER_RELEASE_ON_COMPLETE)
-    out = {"workload": f"8 resident scenes {W.SERVE_SCENES} (config-1 distributions); Poisson arrivals; each request: "
-                       f"seeded camera, depth {W.SERVE_DEPTH_SIZES[0]}..{W.SERVE_DEPTH_SIZES[-1]} rasterised on the "
+    out = {"workload": f"8 resident scenes {W.SERVE_SCENES} (positions U[-500,500]^3, scales U[0.5,2], 256 surfaces "
+                       f"x 8 LODs); Poisson arrivals; each request: seeded camera, depth {W.SERVE_DEPTH_SIZES[0]}..{W.SERVE_DEPTH_SIZES[-1]} rasterised on the "
                        f"device from its seed ({W.SERVE_OCCLUDERS} occluders), own pyramid, Hi-Z cull of the whole "
                        f"scene; 64 GB HBM budget; 4 batches in flight, <= 64 requests per batch",
            "rates": []}

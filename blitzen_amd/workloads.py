@@ -158,7 +158,8 @@ SERVE_OCCLUDERS = 128
 
 
 def served_scenes(ctx: bz.Context, seed: int = SERVE_SEED, sizes=SERVE_SCENES):
-    """The resident scenes of the serving workload (config-1 distributions). Returns [(handle, n_objects)]."""
+    """The resident scenes of the serving workload (positions U[-500,500]^3, scales U[0.5,2], 256 surfaces x 8 LODs).
+    Returns [(handle, n_objects)]."""
     out = []
     for k, n in enumerate(sizes):
         sc = bz.synth_scene(seed + k, n, 256, 8)
