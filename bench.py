@@ -162,8 +162,9 @@ def bench_served(ctx, rates, duration_s: float):
     ctx.server_start(budget, inflight=4, max_batch=64, flags=A.SERVER_RELEASE_ON_COMPLETE)
     out = {"workload": f"8 resident scenes {W.SERVE_SCENES} (positions U[-500,500]^3, scales U[0.5,2], 256 surfaces "
                        f"x 8 LODs); Poisson arrivals; each request: seeded camera, depth "
-                       f"{W.SERVE_DEPTH_SIZES[0]}..{W.SERVE_DEPTH_SIZES[-1]} rasterised on the device from its seed ({W.SERVE_OCCLUDERS} occluders), own pyramid, Hi-Z cull of the whole "
-                       f"scene; 64 GB HBM budget; 4 batches in flight, <= 64 requests per batch",
+                       f"{W.SERVE_DEPTH_SIZES[0]}..{W.SERVE_DEPTH_SIZES[-1]} rasterised on the device from its seed "
+                       f"({W.SERVE_OCCLUDERS} occluders), own pyramid, Hi-Z cull of the whole scene; 64 GB HBM "
+                       f"budget; 4 batches in flight, <= 64 requests per batch",
            "rates": []}
     try:
         warm, offs, _ = W.served_requests(scenes, 2000, 0.02, seed=1)
