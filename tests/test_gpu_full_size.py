@@ -3,7 +3,8 @@
   EARLY + LATE with and without the frame cache, the PLAIN pass and the LOD-instanced output;
 - the 256 batched views (38 250 177 object-view pairs, per-view 512x512 / 1024x1024 / 2048x1024 depth and
   reference-extent pyramids) replayed from one graph;
-- the cluster mode scene of 1 048 576 objects (128 surfaces, 16-128 clusters each) after 1% transform updates.
+- the cluster mode scene of 1 048 576 objects (128 surfaces, 16-128 clusters each) after 1% transform updates;
+- the serving loop over its 8 resident scenes (15 466 496 objects) under 160 Poisson arrivals at 32 000 req/s.
 Needs a GPU."""
 from __future__ import annotations
 
@@ -160,3 +161,44 @@ def test_cluster_mode_1m_objects(ctx):
         np.testing.assert_array_equal(got_c, e_cmds)
     finally:
         cf.free()
+
+
+def test_served_8_resident_scenes(ctx):
+    """The serving workload's 8 resident scenes (15 466 496 objects) under a burst of Poisson arrivals at 32 000
+    req/s: every request's commands equal the oracle's cull of its scene with its own device-rasterised depth."""
+    import ctypes as C
+
+    scenes = W.served_scenes(ctx)
+    try:
+        ctx.server_start(64 << 30, inflight=4, max_batch=64)
+        try:
+            reqs, offs, _ = W.served_requests(scenes, 32000, 0.005, seed=5)
+            tickets, statuses = ctx.submit_schedule(reqs, offs)
+            assert (statuses == A.OK).all()
+            results = []
+            for t, req in zip(tickets, reqs):
+                r = ctx.wait(int(t), timeout_s=60)
+                assert r.status == A.OK
+                n_obj = dict((h, n) for h, n in scenes)[req.scene]
+                cmds, n = ctx.fetch(int(t), n_obj)
+                assert n == r.count
+                results.append((req, H.cmds_u32(cmds)[:n], r.batch_size))
+        finally:
+            ctx.server_stop()
+        assert max(b for _, _, b in results) > 1  # some requests were batched together
+        host = {}
+        for k, (h, n) in enumerate(scenes):
+            host[h] = H.as_tuple(bz.synth_scene(W.SERVE_SEED + k, n, 256, 8))
+        for req, got, _ in results:
+            v = bz.CameraViewData()
+            C.memmove(C.addressof(v), C.addressof(req.view), C.sizeof(v))
+            bw, bh, mips = bz.reference_pyramid_extent(req.depth_w, req.depth_h)
+            v.pyramidWidth, v.pyramidHeight = float(bw), float(bh)
+            depth = bz.synth_depth(req.depth_seed, req.depth_w, req.depth_h, req.occluders, v.proj5, v.zNear)
+            opyr = orc.pyramid(orc.pyramid_build(depth, bw, bh, mips), bw, bh, mips)
+            exp, ne = orc.cull(host[req.scene], H.view_bytes(v), A.PASS_TEMPORAL, opyr)
+            assert got.shape[0] == ne, req.user_tag
+            np.testing.assert_array_equal(got, exp)
+    finally:
+        for h, _ in scenes:
+            ctx.scene_destroy(h)
