@@ -24,7 +24,8 @@
  *                         261-278, 312-372, 497-534, 559-570; blitMLTypes.h:143-193)
  *   orc_cluster_*         PreClusterDrawCull.comp.glsl:13-47, InitialClusterCull.comp.glsl:12-55, plus the
  *                         cluster tests this library defines (the reference has none).
- *   orc_cull_instanced    drawInstCull.cs.hlsl + drawInstCmd.cs.hlsl (LOD-instanced output, deterministic order)
+ *   orc_cull_instanced    HlslShaders/CS/drawInstCull.cs.hlsl:12-42 + drawInstCmd.cs.hlsl:9-38 (LOD-instanced output in
+ *                         a deterministic order; host dx12Draw.cpp:340-413)
  *
  * Points the reference leaves to fixed-function hardware are settled by explicit rules, shared with the
  * HIP kernels:
