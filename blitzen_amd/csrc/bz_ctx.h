@@ -19,7 +19,8 @@ struct Scene {
     uint32_t n_objects = 0, n_transforms = 0, n_surfaces = 0, n_lods = 0, n_clusters = 0;
     uint32_t max_lod_clusters = 0;   // largest LodData.clusterCount (set with the cluster table)
     bool xf_identity = false;
-    uint32_t* obj_surf = nullptr;
+    uint32_t* obj_surf = nullptr;      // surface ids, 32-bit (more than 65 536 surfaces) ...
+    uint16_t* obj_surf16 = nullptr;    // ... or 16-bit (exactly one of the two is set)
     uint32_t* obj_xf = nullptr;
     float4* xf_ps = nullptr;
     float4* xf_q = nullptr;

@@ -11,7 +11,8 @@ namespace bz {
 // Resident scene in HBM, structure-of-arrays. Transforms are split into two 16-byte streams so every
 // per-object load of a wavefront is one fully coalesced 1 KiB access.
 struct SceneDev {
-    const uint32_t* obj_surf;   // [n_objects] RenderObject.surfaceId
+    const uint32_t* obj_surf;   // [n_objects] RenderObject.surfaceId (nullptr when stored as 16-bit)
+    const uint16_t* obj_surf16; // [n_objects] RenderObject.surfaceId when every id < 65536 (else nullptr)
     const uint32_t* obj_xf;     // [n_objects] RenderObject.transformId; nullptr when transformId == objectId
     const float4* xf_ps;        // [n_transforms] pos.xyz, scale
     const float4* xf_q;         // [n_transforms] orientation

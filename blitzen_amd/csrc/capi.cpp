@@ -68,7 +68,7 @@ Pyramid* Ctx::pyramid(bz_pyramid h)
 
 Scene::~Scene()
 {
-    void* ps[] = { obj_surf, obj_xf, xf_ps, xf_q, surf_sphere, surf_lod, lod_error, lod_draw, lod_cluster, vis,
+    void* ps[] = { obj_surf, obj_surf16, obj_xf, xf_ps, xf_q, surf_sphere, surf_lod, lod_error, lod_draw, lod_cluster, vis,
                    cl_sphere, cl_cone, cl_draw, cache.sph, cache.meta, cache.counts };
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -182,6 +182,7 @@ SceneDev Scene::dev() const
 {
     SceneDev d;
     d.obj_surf = obj_surf;
+    d.obj_surf16 = obj_surf16;
     d.obj_xf = obj_xf;
     d.xf_ps = xf_ps;
     d.xf_q = xf_q;
@@ -396,9 +397,14 @@ bz_status bz_scene_create(bz_ctx* ctx, const bz_RenderObject* objects, uint32_t 
     s->n_lods = n_lods;
     s->xf_identity = identity;
     // host SoA staging
-    std::vector<uint32_t> surf_ids(n_objects), xf_ids(identity ? 0 : n_objects);
+    // surface ids as 16-bit when every id fits (any table of at most 65 536 surfaces): 2 B instead of 4 B per object
+    // on every pass over the scene
+    const bool surf16 = n_surfaces <= 65536u;
+    std::vector<uint32_t> surf_ids(surf16 ? 0 : n_objects), xf_ids(identity ? 0 : n_objects);
+    std::vector<uint16_t> surf_ids16(surf16 ? n_objects : 0);
     for (uint32_t i = 0; i < n_objects; ++i) {
-        surf_ids[i] = objects[i].surfaceId;
+        if (surf16) surf_ids16[i] = (uint16_t)objects[i].surfaceId;
+        else surf_ids[i] = objects[i].surfaceId;
         if (!identity) xf_ids[i] = objects[i].transformId;
     }
     std::vector<float4> ps(n_transforms), q(n_transforms), sph(n_surfaces);
@@ -431,7 +437,11 @@ bz_status bz_scene_create(bz_ctx* ctx, const bz_RenderObject* objects, uint32_t 
         return BZ_OK;
     };
     bz_status st;
-    if ((st = up((void**)&s->obj_surf, surf_ids.data(), surf_ids.size() * 4))) return st;
+    if (surf16) {
+        if ((st = up((void**)&s->obj_surf16, surf_ids16.data(), surf_ids16.size() * 2))) return st;
+    } else if ((st = up((void**)&s->obj_surf, surf_ids.data(), surf_ids.size() * 4))) {
+        return st;
+    }
     if (!identity && (st = up((void**)&s->obj_xf, xf_ids.data(), xf_ids.size() * 4))) return st;
     if ((st = up((void**)&s->xf_ps, ps.data(), ps.size() * sizeof(float4)))) return st;
     if ((st = up((void**)&s->xf_q, q.data(), q.size() * sizeof(float4)))) return st;

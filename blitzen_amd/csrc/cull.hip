@@ -126,7 +126,7 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
     for (int k = 0; k < ITEMS; ++k) {
         if (MODE != MODE_EARLY || ALL || s.active[k]) {
             const uint32_t x = XF_ID ? s.gi[k] : S.obj_xf[s.gi[k]];
-            s.surf[k] = S.obj_surf[s.gi[k]];
+            s.surf[k] = S.obj_surf16 ? (uint32_t)S.obj_surf16[s.gi[k]] : S.obj_surf[s.gi[k]];
             s.ps[k] = S.xf_ps[x];
             s.q[k] = S.xf_q[x];
         }

@@ -258,3 +258,22 @@ def test_determinism_repeat(ctx):
         assert n1 == n0
         np.testing.assert_array_equal(again, first)
     ctx.scene_destroy(h)
+
+
+@pytest.mark.parametrize("n_surfaces", [65536, 65537])
+def test_surface_id_width(ctx, n_surfaces):
+    """Surface ids are kept 16-bit when every id fits (up to 65 536 surfaces) and 32-bit otherwise; both give the
+    oracle's commands, including for the largest ids."""
+    n = 150000
+    sc = bz.synth_scene(H.SEED_C1 + n_surfaces, n, n_surfaces, 2)
+    objects = sc.objects.copy()
+    objects["surfaceId"][:1000] = n_surfaces - 1 - np.arange(1000, dtype=np.uint32)  # the top of the id range
+    scene = (objects, sc.transforms, sc.surfaces, sc.lods)
+    h = ctx.scene_create(*scene)
+    v = H.view()
+    v.lodTarget = 1e-4
+    got, ng = gpu_cull(ctx, h, v, A.PASS_PLAIN, n)
+    exp, ne = orc.cull(scene, H.view_bytes(v), A.PASS_PLAIN)
+    assert ng == ne and ne > 1000
+    np.testing.assert_array_equal(got, exp)
+    ctx.scene_destroy(h)
