@@ -41,6 +41,23 @@ __device__ __forceinline__ void reduce_lds(const float* src, uint32_t sw, uint32
     }
 }
 
+// Waits for one flagged per-tile word of the tail, already loaded once into w (the wait is bounded; giving up sets
+// *err), and clears it after reading, so the next build starts from zero.
+__device__ __forceinline__ float tail_finish(const PyrBuildJob& J, uint32_t idx, unsigned long long w, uint32_t* err)
+{
+    uint32_t spins = 0;
+    while (!(w & kTailFlag)) {
+        if (++spins > kTailSpinLimit) {
+            atomicExch(err, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w = __hip_atomic_load(&J.tail[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __hip_atomic_store(&J.tail[idx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __uint_as_float((uint32_t)w);
+}
+
 // Reads one per-tile word for the tail: the owner's own value, or the flagged word another workgroup stored
 // (cleared after reading, so the next build starts from zero). The wait is bounded; giving up sets *err.
 __device__ __forceinline__ float tail_word(const PyrBuildJob& J, uint32_t idx, uint32_t own_tile, float own,
@@ -149,10 +166,42 @@ __device__ void pyramid_tile(const PyrBuildJob& J, uint32_t tile, float* bufA, f
     }
     const float own = src[0];
     __syncthreads();  // every thread has read `own` before bufA is rewritten
-    // the tile image (one texel per tile, level tile_levels - 1) is reduced to the next level while it is collected
     const uint32_t L0 = J.tile_levels - 1u;
     cw = level_dim(J.base_w, L0);
     ch = level_dim(J.base_h, L0);
+    if (n_tiles <= kTileTexels) {
+        // the whole tile image (one texel per tile, level L0) fits the LDS tile: every word of a thread is loaded
+        // before the first wait (one round trip), then the remaining levels are reduced in LDS
+        constexpr uint32_t KW = kTileTexels / kPyrBlock;
+        unsigned long long w[KW];
+#pragma unroll
+        for (uint32_t k = 0; k < KW; ++k) {
+            const uint32_t i = tid + k * kPyrBlock;
+            w[k] = (i < n_tiles && i != tile)
+                       ? __hip_atomic_load(&J.tail[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : 0ull;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < KW; ++k) {
+            const uint32_t i = tid + k * kPyrBlock;
+            if (i < n_tiles) bufA[i] = i == tile ? own : tail_finish(J, i, w[k], err);
+        }
+        __syncthreads();
+        src = bufA;
+        dst = bufB;
+        for (uint32_t L = L0 + 1u; L < J.mips; ++L) {
+            const uint32_t nw = cw > 1u ? cw >> 1 : 1u, nh = ch > 1u ? ch >> 1 : 1u;
+            reduce_lds(src, cw, ch, dst, nw, nh, J.texels + J.level_offset[L], nw, 0u, 0u);
+            __syncthreads();
+            float* tmp = src;
+            src = dst;
+            dst = tmp;
+            cw = nw;
+            ch = nh;
+        }
+        return;
+    }
+    // larger tile images: reduced to the next level while they are collected
     {
         const uint32_t nw = cw > 1u ? cw >> 1 : 1u, nh = ch > 1u ? ch >> 1 : 1u;
         const uint32_t nshift = (uint32_t)__builtin_ctz(nw);
