@@ -172,7 +172,9 @@ BZ_API bz_status bz_scene_create(bz_ctx* ctx,
 BZ_API bz_status bz_scene_set_clusters(bz_ctx* ctx, bz_scene scene, const bz_Cluster* clusters, uint32_t n_clusters);
 BZ_API bz_status bz_scene_destroy(bz_ctx* ctx, bz_scene scene);
 /* Replaces UpdateObjectTransform + UpdateBuffers (vulkanDraw.cpp:816-820, 46-60): scatters n transforms
- * (host arrays) into the resident transform streams, ordered on `stream`. Ids must be < n_transforms. */
+ * (host arrays) into the resident transform streams, ordered on `stream`. Ids must be < n_transforms and distinct.
+ * As in the reference (updates between frames), a cull reading the scene on another stream, including the serving
+ * loop's, is not ordered with the update: order them with the streams, or update between frames. */
 BZ_API bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const uint32_t* transform_ids,
                                      const bz_MeshTransform* transforms, uint32_t n, void* stream);
 /* Caller-filled staging for transform updates, as the reference's persistently mapped staging buffer
