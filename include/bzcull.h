@@ -182,8 +182,10 @@ BZ_API bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const u
  * host arrays of `capacity` transform ids and MeshTransforms (32 B records, the reference layout). The caller fills the
  * first n entries and calls bz_scene_commit_transforms: a kernel on `stream` reads them straight from the pinned slot
  * and scatters them (no host copy or conversion). Two slots are used in turn: the caller fills one while the other's
- * scatter runs; handing a slot out again waits for its previous scatter. Ids within one commit must be distinct. Calling the staging function again before committing returns the same slot.
- * Ids are checked on the device: an id >= n_transforms is skipped and bz_check(stream) reports BZ_E_INVALID. */
+ * scatter runs; handing a slot out again waits for its previous scatter. Ids within one commit must be distinct.
+ * Calling the staging function again before committing returns the same slot (reallocated, without its contents,
+ * when the new capacity is larger). Ids are checked on the device: an id >= n_transforms is skipped and
+ * bz_check(stream) reports BZ_E_INVALID. */
 BZ_API bz_status bz_scene_transform_staging(bz_ctx* ctx, bz_scene scene, uint32_t capacity, uint32_t** ids,
                                             bz_MeshTransform** transforms);
 BZ_API bz_status bz_scene_commit_transforms(bz_ctx* ctx, bz_scene scene, uint32_t n, void* stream);
