@@ -9,7 +9,8 @@
 //            chunk's entries loaded as 64-entry groups, kBatch groups per round trip);
 //   scan:    one workgroup per 64 LODs: exclusive prefix of the histogram over chunks (the chunk's first slot in the
 //            LOD's list) and the counter {lod * per_lod, total};
-//   cmds:    one workgroup: the LODs with instances in ascending order -> 32-byte commands and their number;
+//   cmds:    one extra workgroup of the scatter launch: the LODs with instances in ascending order -> 32-byte commands
+//            and their number;
 //   scatter: one workgroup per chunk x LOD slice: per-wave LOD counts over quarters of the chunk's entries, a hand-over
 //            of each LOD's running slot through the quarters, then ranks among equal LODs with one ballot per bit of
 //            the LOD index; each object written to lod * per_lod + slot.
@@ -137,40 +138,67 @@ __global__ __launch_bounds__(kScanWaves * 64) void inst_scan_kernel(InstancedJob
     }
 }
 
-// One workgroup: commands for the LODs with instances, ascending (drawInstCmd.cs.hlsl's fields).
+// Commands for the LODs with instances, ascending (drawInstCmd.cs.hlsl's fields), and their number, by one workgroup
+// of NT threads: thread t takes the contiguous LODs [t * R, (t + 1) * R), counts those with instances (their counters
+// loaded 8 at a time), a block scan gives its first command slot, and it writes its commands in order.
+// wsum: NT / 64 LDS words.
+template <uint32_t NT>
+__device__ __forceinline__ void build_commands(const InstancedJob& J, uint32_t* wsum)
+{
+    constexpr uint32_t NW = NT / 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t R = (J.n_lods + NT - 1u) / NT;
+    const uint32_t lb = tid * R < J.n_lods ? tid * R : J.n_lods;
+    const uint32_t le = lb + R < J.n_lods ? lb + R : J.n_lods;
+    uint32_t mine = 0;
+    for (uint32_t l0 = lb; l0 < le; l0 += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) c[j] = l0 + j < le ? J.counters[2u * (l0 + j) + 1u] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) mine += c[j] != 0u ? 1u : 0u;
+    }
+    uint32_t incl = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t pos = incl - mine, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        pos += w < wave ? wsum[w] : 0u;
+        total += wsum[w];
+    }
+    if (tid == 0) *J.count = total;
+    for (uint32_t l0 = lb; l0 < le; l0 += 8) {
+        uint32_t c[8];
+        uint2 d[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) c[j] = l0 + j < le ? J.counters[2u * (l0 + j) + 1u] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) d[j] = c[j] != 0u ? J.lod_draw[l0 + j] : make_uint2(0u, 0u);
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            if (c[j] == 0u) continue;
+            if (pos < J.capacity) {
+                const uint32_t l = l0 + j;
+                uint4* cmd = reinterpret_cast<uint4*>(J.cmds + (size_t)pos * 8u);
+                cmd[0] = make_uint4(l * J.per_lod, d[j].x, c[j] < J.per_lod ? c[j] : J.per_lod, d[j].y);  // objId = offset
+                cmd[1] = make_uint4(0u, 0u, 0u, 0u);  // vertexOffset, firstInstance, padding
+            }
+            ++pos;
+        }
+    }
+}
+
+// The commands alone (a range without objects: no scatter launch follows).
 __global__ __launch_bounds__(1024) void inst_cmd_kernel(InstancedJob J)
 {
     __shared__ uint32_t wsum[16];
-    __shared__ uint32_t s_base;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    if (tid == 0) s_base = 0;
-    __syncthreads();
-    for (uint32_t l0 = 0; l0 < J.n_lods; l0 += 1024) {
-        const uint32_t l = l0 + tid;
-        const uint32_t cnt = l < J.n_lods ? J.counters[2u * l + 1u] : 0u;
-        const bool has = cnt != 0u;
-        const unsigned long long m = __ballot(has);
-        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t before = s_base, total = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < 16; ++w) {
-            before += w < wave ? wsum[w] : 0u;
-            total += wsum[w];
-        }
-        const uint32_t pos = before + rank;
-        if (has && pos < J.capacity) {
-            const uint2 d = J.lod_draw[l];
-            uint4* cmd = reinterpret_cast<uint4*>(J.cmds + (size_t)pos * 8u);
-            cmd[0] = make_uint4(l * J.per_lod, d.x, cnt < J.per_lod ? cnt : J.per_lod, d.y);  // objId = offset
-            cmd[1] = make_uint4(0u, 0u, 0u, 0u);  // vertexOffset, firstInstance, padding
-        }
-        __syncthreads();
-        if (tid == 0) s_base += total;
-        __syncthreads();
-    }
-    if (tid == 0) *J.count = s_base;
+    build_commands<1024>(J, wsum);
 }
 
 // One workgroup (4 waves) per chunk x LOD slice of kScatterSlice LODs. The chunk's entries form one sequence of
@@ -188,6 +216,11 @@ __global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(Instan
 {
     __shared__ uint32_t slot[kScatterWaves][kScatterSlice];  // per wave: LOD -> count (1), then next slot (2, 3)
     __shared__ uint32_t running[kScatterSlice];              // per LOD: the next slot after the windows so far
+    __shared__ uint32_t cmd_wsum[kScatterWaves];
+    if (blockIdx.x == J.n_chunks) {  // one extra workgroup writes the commands (final counters since the scan)
+        if (blockIdx.y == 0) build_commands<kScatterWaves * 64>(J, cmd_wsum);
+        return;
+    }
     const uint32_t c = blockIdx.x, l0 = blockIdx.y * kScatterSlice, tid = threadIdx.x, lane = tid & 63u,
                    wave = tid >> 6;
     const uint32_t l1 = l0 + kScatterSlice < J.n_lods ? l0 + kScatterSlice : J.n_lods;
@@ -293,11 +326,12 @@ hipError_t launch_instanced(const InstancedJob& J, const CompactScratch& s, uint
     hipLaunchKernelGGL(inst_scan_kernel, dim3((J.n_lods + 63) / 64), dim3(kScanWaves * 64), 0, st, J, hist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(inst_cmd_kernel, dim3(1), dim3(1024), 0, st, J);
-    e = hipGetLastError();
-    if (e != hipSuccess || !J.n_chunks) return e;
+    if (!J.n_chunks) {
+        hipLaunchKernelGGL(inst_cmd_kernel, dim3(1), dim3(1024), 0, st, J);
+        return hipGetLastError();
+    }
     const uint32_t scatter_slices = (J.n_lods + kScatterSlice - 1) / kScatterSlice;
-    hipLaunchKernelGGL(inst_scatter_kernel, dim3(J.n_chunks, scatter_slices), dim3(kScatterWaves * 64), 0, st, J,
+    hipLaunchKernelGGL(inst_scatter_kernel, dim3(J.n_chunks + 1u, scatter_slices), dim3(kScatterWaves * 64), 0, st, J,
                        (const uint32_t*)s.counts, (const uint2*)s.staging, (const uint32_t*)hist);
     return hipGetLastError();
 }

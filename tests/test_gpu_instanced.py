@@ -94,3 +94,14 @@ def test_instanced_more_than_64_tiles_per_chunk(ctx):
     finally:
         ctx.set_tuning(0, old.block, old.items, old.grid_per_cu)
     ctx.scene_destroy(h)
+
+
+def test_instanced_scene_without_objects(ctx):
+    """No objects: no commands, every counter {lod * per_lod, 0}, no list slot written."""
+    sc = bz.synth_scene(H.SEED_C0, 16, 8, 3)
+    scene = (sc.objects[:0].copy(), sc.transforms, sc.surfaces, sc.lods)
+    h = ctx.scene_create(*scene)
+    got = run(ctx, h, scene, H.view(), 5)
+    check(got, orc.cull_instanced(scene, H.view_bytes(H.view()), 5))
+    assert got[3] == 0 and (got[1][:, 1] == 0).all() and (got[0] == 0xFFFFFFFF).all()
+    ctx.scene_destroy(h)
