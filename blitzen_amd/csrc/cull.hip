@@ -89,9 +89,31 @@ struct Streams {
     float4 ps[ITEMS], q[ITEMS];
 };
 
+// Loads of the per-object streams. NT: marked non-temporal (streaming, evicted first), for an EARLY pass that records
+// a frame cache: its LATE pass reads the records, not these streams again (measured 4% faster per frame). Without the
+// cache the LATE pass re-reads the same streams and finds part of them still in the last-level cache, which the hint
+// would give up (measured 2.5% slower per frame), so every other pass keeps the default policy.
+typedef float nt_float4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ float4 stream_load(const float4* p)
+{
+    if constexpr (NT) {
+        const nt_float4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_float4*>(p));
+        return make_float4(v.x, v.y, v.z, v.w);
+    } else {
+        return *p;
+    }
+}
+template <bool NT, class T>
+__device__ __forceinline__ T stream_load(const T* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // Tile indices and the visibility stream (EARLY, LATE). Jobs always hold at least one object (empty ranges are
 // answered on the host), so clamped indices are always valid.
-template <int BLOCK, int ITEMS, int MODE>
+template <int BLOCK, int ITEMS, int MODE, bool NT = false>
 __device__ __forceinline__ void issue_vis(const CullJob& J, uint32_t tile_in_job, Streams<ITEMS>& s)
 {
     const uint32_t base = tile_in_job * (uint32_t)(BLOCK * ITEMS);
@@ -105,7 +127,7 @@ __device__ __forceinline__ void issue_vis(const CullJob& J, uint32_t tile_in_job
     }
     if (MODE == MODE_EARLY || MODE == MODE_LATE) {  // J.n >= 1: clamped indices are in range
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) s.vis[k] = J.vis_in[s.gi[k]];
+        for (int k = 0; k < ITEMS; ++k) s.vis[k] = stream_load<NT>(J.vis_in + s.gi[k]);
     }
 }
 
@@ -125,10 +147,12 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         if (MODE != MODE_EARLY || ALL || s.active[k]) {
-            const uint32_t x = XF_ID ? s.gi[k] : S.obj_xf[s.gi[k]];
-            s.surf[k] = S.obj_surf16 ? (uint32_t)S.obj_surf16[s.gi[k]] : S.obj_surf[s.gi[k]];
-            s.ps[k] = S.xf_ps[x];
-            s.q[k] = S.xf_q[x];
+            constexpr bool NT = MODE == MODE_EARLY && ALL;
+            const uint32_t x = XF_ID ? s.gi[k] : stream_load<NT>(S.obj_xf + s.gi[k]);
+            s.surf[k] = S.obj_surf16 ? (uint32_t)stream_load<NT>(S.obj_surf16 + s.gi[k])
+                                     : stream_load<NT>(S.obj_surf + s.gi[k]);
+            s.ps[k] = stream_load<NT>(S.xf_ps + x);
+            s.q[k] = stream_load<NT>(S.xf_q + x);
         }
     }
 }
@@ -136,7 +160,7 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
 template <int BLOCK, int ITEMS, int MODE, bool XF_ID, bool ALL = false>
 __device__ __forceinline__ void issue_tile(const CullJob& J, uint32_t tile_in_job, Streams<ITEMS>& s)
 {
-    issue_vis<BLOCK, ITEMS, MODE>(J, tile_in_job, s);
+    issue_vis<BLOCK, ITEMS, MODE, MODE == MODE_EARLY && ALL>(J, tile_in_job, s);
     issue_objects<ITEMS, MODE, XF_ID, ALL>(J, s);
 }
 
