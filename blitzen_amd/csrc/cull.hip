@@ -89,10 +89,11 @@ struct Streams {
     float4 ps[ITEMS], q[ITEMS];
 };
 
-// Loads of the per-object streams. NT: marked non-temporal (streaming, evicted first), for an EARLY pass that records
-// a frame cache: its LATE pass reads the records, not these streams again (measured 4% faster per frame). Without the
-// cache the LATE pass re-reads the same streams and finds part of them still in the last-level cache, which the hint
-// would give up (measured 2.5% slower per frame), so every other pass keeps the default policy.
+// Loads of the per-object streams. NT: marked non-temporal (streaming, evicted first), for the passes that read each
+// stream once and gather no Hi-Z texels: an EARLY pass recording a frame cache (its LATE pass reads the records, not
+// these streams again: 4% faster per frame), PLAIN and ONPC (4%). Without the cache the LATE pass re-reads the same
+// streams and finds part of them still in the last-level cache, which the hint would give up (2.5% slower per frame),
+// and passes with a Hi-Z test keep their texels cached better without it, so those keep the default policy.
 typedef float nt_float4 __attribute__((ext_vector_type(4)));
 template <bool NT>
 __device__ __forceinline__ float4 stream_load(const float4* p)
@@ -134,7 +135,7 @@ __device__ __forceinline__ void issue_vis(const CullJob& J, uint32_t tile_in_job
 // The surface-id and transform streams: for every object, or in EARLY only for last-frame-visible ones (which waits
 // for the visibility stream first).
 // ALL (EARLY recording a frame cache): every object's streams are loaded, not only the last-frame-visible ones.
-template <int ITEMS, int MODE, bool XF_ID, bool ALL = false>
+template <int ITEMS, int MODE, bool XF_ID, bool ALL = false, bool NT = false>
 __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& s)
 {
     const SceneDev& S = J.scene;
@@ -147,7 +148,6 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
         if (MODE != MODE_EARLY || ALL || s.active[k]) {
-            constexpr bool NT = MODE == MODE_EARLY && ALL;
             const uint32_t x = XF_ID ? s.gi[k] : stream_load<NT>(S.obj_xf + s.gi[k]);
             s.surf[k] = S.obj_surf16 ? (uint32_t)stream_load<NT>(S.obj_surf16 + s.gi[k])
                                      : stream_load<NT>(S.obj_surf + s.gi[k]);
@@ -157,11 +157,11 @@ __device__ __forceinline__ void issue_objects(const CullJob& J, Streams<ITEMS>& 
     }
 }
 
-template <int BLOCK, int ITEMS, int MODE, bool XF_ID, bool ALL = false>
+template <int BLOCK, int ITEMS, int MODE, bool XF_ID, bool ALL = false, bool NT = false>
 __device__ __forceinline__ void issue_tile(const CullJob& J, uint32_t tile_in_job, Streams<ITEMS>& s)
 {
-    issue_vis<BLOCK, ITEMS, MODE, MODE == MODE_EARLY && ALL>(J, tile_in_job, s);
-    issue_objects<ITEMS, MODE, XF_ID, ALL>(J, s);
+    issue_vis<BLOCK, ITEMS, MODE, NT>(J, tile_in_job, s);
+    issue_objects<ITEMS, MODE, XF_ID, ALL, NT>(J, s);
 }
 
 // Small tables into LDS (surface spheres and LOD ranges, LOD errors) plus the pyramid's level offsets. All loads of a
@@ -428,11 +428,12 @@ __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_j
                                             uint2* staging, TileSmem<BLOCK, ITEMS>& sm, uint32_t* ccounts = nullptr)
 {
     constexpr bool lds_tab = LDS_TAB;
+    constexpr bool NT = MODE == MODE_OTHER && !HIZ;  // PLAIN / ONPC: streamed once, no Hi-Z
     Streams<ITEMS> s;
-    issue_vis<BLOCK, ITEMS, MODE>(J, tile_in_job, s);
-    if (MODE != MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID>(J, s);
+    issue_vis<BLOCK, ITEMS, MODE, NT>(J, tile_in_job, s);
+    if (MODE != MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID, false, NT>(J, s);
     stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
-    if (MODE == MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID>(J, s);
+    if (MODE == MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID, false, NT>(J, s);
     __syncthreads();  // staged tables visible
     process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, CL>(J, gtile, s, counts, staging, ccounts, sm, NoPrefetch{});
 }
@@ -445,10 +446,11 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
     constexpr bool lds_tab = LDS_TAB;
+    constexpr bool NT = (MODE == MODE_EARLY && CACHE) || (MODE == MODE_OTHER && !HIZ);
     const uint32_t G = gridDim.x;
     uint32_t t = blockIdx.x;
     Streams<ITEMS> A, B;
-    issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE>(J, t, A);
+    issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, t, A);
     stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
     __syncthreads();
     while (true) {
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
             const uint32_t tn = t + G;
             // past the last tile the current one is issued again (L2 hits, discarded): no branch around the loads
             process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, A, counts, staging, nullptr, sm, [&]() {
-                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE>(J, tn < J.n_tiles ? tn : t, B);
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < J.n_tiles ? tn : t, B);
             }, fc);
             t = tn;
             if (t >= J.n_tiles) break;
@@ -464,7 +466,7 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
         {
             const uint32_t tn = t + G;
             process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, B, counts, staging, nullptr, sm, [&]() {
-                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE>(J, tn < J.n_tiles ? tn : t, A);
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < J.n_tiles ? tn : t, A);
             }, fc);
             t = tn;
             if (t >= J.n_tiles) break;
