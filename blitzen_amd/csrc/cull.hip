@@ -838,10 +838,11 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
             round(sp2, mt2, have);
         }
     }
+    // the visibility is next read by the following frame's EARLY pass: stored non-temporal (1 us faster here)
 #pragma unroll
     for (uint32_t k = 0; k < OPT; ++k) {
         const uint32_t i = k * BLOCK + tid;
-        if (i < nt) J.vis_out[obj0 + i] = (s_new[i >> 5] >> (i & 31u)) & 1u;
+        if (i < nt) __builtin_nontemporal_store((s_new[i >> 5] >> (i & 31u)) & 1u, J.vis_out + obj0 + i);
     }
     if (tid == 0) counts[st] = s_base;
 }
