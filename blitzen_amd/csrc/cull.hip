@@ -575,7 +575,7 @@ hipError_t launch_cluster_objects_t(const CullJob& job, const CompactScratch& s,
 // ------------------------------------------------------------------------------------------------ emit
 constexpr uint32_t kEmitBlock = 256;
 constexpr uint32_t kEmitTiles = kEmitBlock / 64;  // one tile per wave
-constexpr uint32_t kEmitUnroll = 4;               // entries per lane in flight
+constexpr uint32_t kEmitUnroll = 2;               // entries per lane in flight (a tile emits ~70 on average)
 
 struct EmitSmem {
     uint32_t red[kEmitBlock / 64];
