@@ -42,9 +42,7 @@ def demangle_short(name: str) -> str:
             if args.startswith("I"):
                 vals = []
                 for tok in args[1:].split("E"):
-                    if tok.startswith("Li"):
-                        vals.append(tok[2:])
-                    elif tok.startswith("Lb"):
+                    if tok.startswith(("Li", "Lj", "Lb")):  # int, unsigned, bool template arguments
                         vals.append(tok[2:])
                     else:
                         break
@@ -96,7 +94,7 @@ def late_pass_values(counters):
     return out
 
 
-PYRAMID = re.compile(r"pyramid_kernel\(|14pyramid_kernelE")
+PYRAMID = re.compile(r"pyramid_kernel[(<]|14pyramid_kernel[EI]")  # plain or templated (block size) symbol
 
 
 def frame_values(counters):
