@@ -163,7 +163,7 @@ struct ClusterScratch {
 };
 
 hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, uint32_t block,
-                                  uint32_t items, uint32_t n_cu, uint32_t per_cu, hipStream_t st);
+                                  uint32_t items, hipStream_t st);
 hipError_t launch_cluster_pass(const ClusterJob& job, const CompactScratch& cs, const ClusterScratch& xs,
                                hipStream_t st);
 

@@ -1001,7 +1001,7 @@ bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* str
     cj.hiz = oj.hiz;
     cj.n_tiles = oj.n_tiles;
     cj.tile_size = t.block * t.items;
-    hipError_t e = launch_cluster_objects(oj, sc->cs, sc->xs.ccounts, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
+    hipError_t e = launch_cluster_objects(oj, sc->cs, sc->xs.ccounts, t.block, t.items, st);
     if (e == hipSuccess) e = launch_cluster_pass(cj, sc->cs, sc->xs, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "cluster cull launch");
     return BZ_OK;

@@ -4,7 +4,7 @@
 // ClusterDispatchData per cluster) and InitialClusterCull.comp.glsl:12-55 (one draw per cluster). The reference
 // reads the expanded count back to the host between the passes (vulkanDraw.cpp:874-905); here everything stays on
 // the device and the order is deterministic:
-//   object pass (cull.hip, decide_kernel with CL): per object tile, the emitting objects (objectId, LOD) in object
+//   object pass (cull.hip, decide_cluster_kernel): per object tile, the emitting objects (objectId, LOD) in object
 //       order and the tile's cluster total;
 //   expand:  one workgroup per object tile. Its first record is the sum of the earlier tiles' totals; it writes the
 //       tile's records in (objectId, clusterId) order, tests every record's cluster (frustum, backface cone, Hi-Z)

@@ -422,10 +422,10 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
     __syncthreads();  // woff / cwoff are rewritten by the next tile
 }
 
-// One tile, loads and decision (non-persistent form, used by the batched kernel).
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB>
+// One tile, loads and decision (non-persistent form, used by the batched kernel and the cluster object pass).
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CL = false>
 __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_job, uint32_t gtile, uint32_t* counts,
-                                            uint2* staging, TileSmem<BLOCK, ITEMS>& sm)
+                                            uint2* staging, TileSmem<BLOCK, ITEMS>& sm, uint32_t* ccounts = nullptr)
 {
     constexpr bool lds_tab = LDS_TAB;
     constexpr bool NT = MODE == MODE_OTHER && !HIZ;  // PLAIN / ONPC: streamed once, no Hi-Z
@@ -435,17 +435,14 @@ __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_j
     stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
     if (MODE == MODE_EARLY) issue_objects<ITEMS, MODE, XF_ID, false, NT>(J, s);
     __syncthreads();  // staged tables visible
-    process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB>(J, gtile, s, counts, staging, nullptr, sm, NoPrefetch{});
+    process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, CL>(J, gtile, s, counts, staging, ccounts, sm, NoPrefetch{});
 }
 
 // Persistent decide for one view: workgroup b takes tiles b, b + G, b + 2G, ... (no workgroup waits on another, so
 // any residency is safe). Tables are staged once; the next tile's streams are issued while the current one is
 // decided (two register sets, alternating, so no register copy waits on a load in flight).
-// CL: the cluster object pass (PreClusterDrawCull.comp.glsl:13-47), which also writes each tile's sum of the
-// emitting objects' LOD cluster counts to ccounts.
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false, bool CL = false>
-__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging, FrameCacheDev fc,
-                                                       uint32_t* ccounts)
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false>
+__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging, FrameCacheDev fc)
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
     constexpr bool lds_tab = LDS_TAB;
@@ -463,7 +460,7 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
             // object, so each stream costs one cache line (discarded). A branch around the loads instead was 2.7 us
             // slower per pass, re-issuing the current tile 2.2 us slower (with non-temporal loads its lines are
             // fetched again).
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, CL, CACHE>(J, t, A, counts, staging, ccounts, sm, [&]() {
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, A, counts, staging, nullptr, sm, [&]() {
                 issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < J.n_tiles ? tn : J.n_tiles, B);
             }, fc);
             t = tn;
@@ -471,7 +468,7 @@ __global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* coun
         }
         {
             const uint32_t tn = t + G;
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, CL, CACHE>(J, t, B, counts, staging, ccounts, sm, [&]() {
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, B, counts, staging, nullptr, sm, [&]() {
                 issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < J.n_tiles ? tn : J.n_tiles, A);
             }, fc);
             t = tn;
@@ -537,6 +534,45 @@ __global__ __launch_bounds__(BLOCK) void decide_batch_kernel(const CullJob* __re
         BZ_DECIDE_TILE(MODE_OTHER, false)
     }
 #undef BZ_DECIDE_TILE
+}
+
+// Cluster object pass (PreClusterDrawCull.comp.glsl:13-47): the PLAIN (or, with a pyramid, TEMPORAL) decision of
+// every object, its LOD, and per tile the number of emitting objects (counts), their (objectId, LOD) list (staging)
+// and the sum of their LOD cluster counts (ccounts). One workgroup per tile.
+template <int BLOCK, int ITEMS, bool HIZ, bool XF_ID, bool LDS_TAB>
+__global__ __launch_bounds__(BLOCK) void decide_cluster_kernel(CullJob J, uint32_t* counts, uint2* staging,
+                                                               uint32_t* ccounts)
+{
+    __shared__ TileSmem<BLOCK, ITEMS> sm;
+    decide_tile<BLOCK, ITEMS, MODE_OTHER, HIZ, XF_ID, LDS_TAB, true>(J, blockIdx.x, blockIdx.x, counts, staging, sm,
+                                                                     ccounts);
+}
+
+template <int BLOCK, int ITEMS>
+hipError_t launch_cluster_objects_t(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, hipStream_t st)
+{
+    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
+    const bool xf_id = job.scene.obj_xf == nullptr;
+    const dim3 g(job.n_tiles), b(BLOCK);
+#define BZ_CL_LAUNCH(HZ)                                                                                           \
+    if (xf_id) {                                                                                                   \
+        if (lds) hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, true, true>), g, b, 0, st, job,      \
+                                    s.counts, s.staging, ccounts);                                                 \
+        else hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, true, false>), g, b, 0, st, job,         \
+                                s.counts, s.staging, ccounts);                                                     \
+    } else {                                                                                                       \
+        if (lds) hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, false, true>), g, b, 0, st, job,     \
+                                    s.counts, s.staging, ccounts);                                                 \
+        else hipLaunchKernelGGL((decide_cluster_kernel<BLOCK, ITEMS, HZ, false, false>), g, b, 0, st, job,        \
+                                s.counts, s.staging, ccounts);                                                     \
+    }
+    if (job.hiz) {
+        BZ_CL_LAUNCH(true)
+    } else {
+        BZ_CL_LAUNCH(false)
+    }
+#undef BZ_CL_LAUNCH
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------------ emit
@@ -828,15 +864,15 @@ hipError_t launch_late_cached_t(const CullJob& job, const CompactScratch& s, con
 }
 
 // Persistent grid: `per_cu` workgroups per CU, or (per_cu == 0) as many as are resident at once per CU.
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false, bool CL = false>
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false>
 void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st,
-                   const FrameCacheDev& fc = FrameCacheDev{}, uint32_t* ccounts = nullptr)
+                   const FrameCacheDev& fc = FrameCacheDev{})
 {
     static int resident = 0;
     if (resident == 0) {
         int r = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE, CL>, BLOCK, 0) != hipSuccess ||
+                &r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE>, BLOCK, 0) != hipSuccess ||
             r < 1)
             r = 1;
         resident = r;
@@ -844,42 +880,8 @@ void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t*
     uint64_t grid = (uint64_t)n_cu * (per_cu ? per_cu : (uint32_t)resident);
     if (grid > job.n_tiles) grid = job.n_tiles;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE, CL>), dim3((uint32_t)grid),
-                       dim3(BLOCK), 0, st, job, counts, staging, fc, ccounts);
-}
-
-// Cluster object pass: the PLAIN (or, with a pyramid, TEMPORAL) decision of every object, its LOD, and per tile the
-// number of emitting objects (counts), their (objectId, LOD) list (staging) and the sum of their LOD cluster counts
-// (ccounts), on the persistent grid.
-template <int BLOCK, int ITEMS>
-hipError_t launch_cluster_objects_t(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, uint32_t n_cu,
-                                    uint32_t per_cu, hipStream_t st)
-{
-    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
-    const bool xf_id = job.scene.obj_xf == nullptr;
-    const FrameCacheDev none{};
-#define BZ_CL_LAUNCH(HZ, XF, LT)                                                                                   \
-    launch_decide<BLOCK, ITEMS, MODE_OTHER, HZ, XF, LT, false, true>(job, n_cu, per_cu, s.counts, s.staging, st,  \
-                                                                     none, ccounts)
-    if (job.hiz) {
-        if (xf_id) {
-            if (lds) BZ_CL_LAUNCH(true, true, true);
-            else BZ_CL_LAUNCH(true, true, false);
-        } else {
-            if (lds) BZ_CL_LAUNCH(true, false, true);
-            else BZ_CL_LAUNCH(true, false, false);
-        }
-    } else {
-        if (xf_id) {
-            if (lds) BZ_CL_LAUNCH(false, true, true);
-            else BZ_CL_LAUNCH(false, true, false);
-        } else {
-            if (lds) BZ_CL_LAUNCH(false, false, true);
-            else BZ_CL_LAUNCH(false, false, false);
-        }
-    }
-#undef BZ_CL_LAUNCH
-    return hipGetLastError();
+    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE>), dim3((uint32_t)grid),
+                       dim3(BLOCK), 0, st, job, counts, staging, fc);
 }
 
 template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool CACHE = false>
@@ -978,10 +980,10 @@ hipError_t launch_late_cached(const CullJob& job_in, const CompactScratch& s, co
 }
 
 hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, uint32_t* ccounts, uint32_t block,
-                                  uint32_t items, uint32_t n_cu, uint32_t per_cu, hipStream_t st)
+                                  uint32_t items, hipStream_t st)
 {
 #define X(B, I) \
-    if (block == B && items == I) return launch_cluster_objects_t<B, I>(job, s, ccounts, n_cu, per_cu, st);
+    if (block == B && items == I) return launch_cluster_objects_t<B, I>(job, s, ccounts, st);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;
