@@ -370,6 +370,8 @@ bz_status bz_scene_create(bz_ctx* ctx, const bz_RenderObject* objects, uint32_t 
 {
     if (!ctx || !out) return BZ_E_INVALID;
     *out = 0;
+    if (n_objects > BZ_MAX_OBJECTS)
+        return ctx->fail(BZ_E_INVALID, "more than BZ_MAX_OBJECTS objects in one scene");
     if ((n_objects && !objects) || (n_transforms && !transforms) || (n_surfaces && !surfaces) || (n_lods && !lods))
         return ctx->fail(BZ_E_INVALID, "NULL array with non-zero count");
     if (n_surfaces == 0 || n_lods == 0 || n_transforms == 0)

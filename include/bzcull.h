@@ -162,6 +162,7 @@ BZ_API uint32_t bz_abi_version(void);
  * (vulkanRendererSetup.cpp:313-363, 365-647): uploads the host arrays once, keeps them resident in HBM
  * as structure-of-arrays streams, and zero-fills the visibility buffer (vulkanRendererSetup.cpp:349).
  * ------------------------------------------------------------------------------------------- */
+#define BZ_MAX_OBJECTS 0x80000000u /* objects per scene (2^31): tile, slot and record offsets are 32-bit */
 BZ_API bz_status bz_scene_create(bz_ctx* ctx,
                           const bz_RenderObject* objects, uint32_t n_objects,
                           const bz_MeshTransform* transforms, uint32_t n_transforms,

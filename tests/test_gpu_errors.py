@@ -99,6 +99,13 @@ def test_other_entry_points_reject(env):
     assert lib.bz_pyramid_create(ctx.handle, 64, 32, 17, C.byref(out)) == A.E_INVALID
     assert lib.bz_pyramid_destroy(ctx.handle, 55555) == A.E_INVALID
     assert lib.bz_scene_destroy(ctx.handle, 55555) == A.E_INVALID
+    # more objects than one scene holds: rejected before any array is read
+    one = H.c1_scene(1)
+    sc_out = C.c_uint64()
+    assert lib.bz_scene_create(ctx.handle, A.ptr(one.objects), 0x80000001, A.ptr(one.transforms), 1,
+                               A.ptr(one.surfaces), len(one.surfaces), A.ptr(one.lods), len(one.lods),
+                               C.byref(sc_out)) == A.E_INVALID
+    assert sc_out.value == 0
     assert lib.bz_pyramid_build(ctx.handle, 55555, cmds.ptr, 128, 64, None) == A.E_INVALID
     ids = np.array([10, 6000], np.uint32)  # 6000 is past the transform table
     xf = np.zeros(2, A.MESH_TRANSFORM)
