@@ -233,7 +233,8 @@ def bench_clusters(ctx, stream, steps: int, warmup: int):
     ms = statistics.median([a.elapsed_ms(b) for a, b in evs])
     out = {"workload": "1,048,576 objects over 128 surfaces x 8 LODs with 16-128 clusters of 64 triangles (LOD k "
                        "owns max(1, c >> 2k)); per frame 1% seeded transform updates (10,485 x 32 B uploaded) + object "
-                       "pass + expansion + cluster frustum/Hi-Z/cone tests against a 1024x512 11-mip pyramid",
+                       "pass (frustum + LOD, as PreClusterDrawCull) + expansion + cluster frustum/Hi-Z/cone tests "
+                       "against a 1024x512 11-mip pyramid",
            "frame_ms_p50": round(ms, 4), "frame_wall_ms": round(wall * 1e3, 4),
            "objects_per_s": round(cf.n_objects / (ms * 1e-3), 1),
            "cluster_records": records, "cluster_draws": draws,

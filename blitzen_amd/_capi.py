@@ -121,6 +121,7 @@ PASS_EARLY, PASS_LATE, PASS_PLAIN, PASS_TEMPORAL, PASS_ONPC = 0, 1, 2, 3, 4
 CULL_EARLY_HIZ = 0x1
 CULL_FRAME_CACHE = 0x2
 CLUSTER_FRUSTUM, CLUSTER_HIZ, CLUSTER_CONE = 0x1, 0x2, 0x4
+CLUSTER_OBJECT_HIZ = 0x1  # bz_cluster_cull_args.flags
 SERVER_RELEASE_ON_COMPLETE = 0x1
 EXTSEM_BINARY = 0
 EXTSEM_TIMELINE = 1
@@ -144,6 +145,7 @@ SIGNATURES = {
     "bz_scene_set_visibility": (_I32, [_VP, _U64, _VP, _U32, _VP]),
     "bz_scene_get_visibility": (_I32, [_VP, _U64, _VP, _U32, _VP]),
     "bz_scene_visibility_ptr": (_I32, [_VP, _U64, C.POINTER(_VP)]),
+    "bz_scene_frame_cache_stats": (_I32, [_VP, _U64, C.POINTER(_U64), C.POINTER(_U64)]),
     "bz_scene_info": (_I32, [_VP, _U64, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32),
                              C.POINTER(_U64)]),
     "bz_pyramid_create": (_I32, [_VP, _U32, _U32, _U32, C.POINTER(_U64)]),

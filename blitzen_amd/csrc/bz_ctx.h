@@ -35,7 +35,8 @@ struct Scene {
     uint2* cl_draw = nullptr;
     uint64_t bytes = 0;
     uint64_t xf_version = 0;            // bumped by every transform update (invalidates the frame cache)
-    // frame cache: the frustum survivors of the last EARLY pass that recorded one (BZ_CULL_FRAME_CACHE)
+    // frame cache: the frustum survivors of the last EARLY pass that recorded one (BZ_CULL_FRAME_CACHE); valid for
+    // the same view, object range and transforms (the visibility is read by the LATE pass itself)
     struct FrameCache {
         float4* sph = nullptr;
         uint2* meta = nullptr;
@@ -46,7 +47,7 @@ struct Scene {
         float view_key[19] = {};        // view matrix rows, frustum constants and lodTarget the survivors used
         uint32_t begin = 0, n = 0, block = 0, items = 0;
         uint64_t xf_version = 0;
-        const uint32_t* vis_in = nullptr;  // the last-frame visibility the EARLY pass read (recorded per survivor)
+        uint64_t recorded = 0, reused = 0;  // EARLY passes that recorded, LATE passes that read the records
     } cache;
     // transform-update staging (bz_scene_transform_staging, also behind bz_scene_update_transforms): two pinned slots
     // [ids | 32-byte MeshTransforms], used in turn
@@ -167,6 +168,8 @@ struct Ctx {
     bz_status upload_jobs(StreamScratch* sc, const CullJob* jobs, uint32_t n, hipStream_t st);
     bz_status upload_pjobs(StreamScratch* sc, const PyrBuildJob* jobs, uint32_t n, hipStream_t st);
     bz_status build_pyramid(Pyramid* p, const float* depth, uint32_t w, uint32_t h, hipStream_t st);
+    // bz_cull; with `fixed`, that compaction scratch is used and nothing is allocated (bz_cull_noalloc)
+    bz_status cull(const bz_cull_args* a, hipStream_t st, const CompactScratch* fixed);
 };
 
 ViewConsts make_view_consts(const bz_CameraViewData& v);

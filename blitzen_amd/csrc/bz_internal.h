@@ -68,12 +68,12 @@ constexpr int kMaxMips = 16;
 // Launchers (cull.hip). Each returns hipSuccess or the launch error.
 // The decide kernel is persistent: n_cu * per_cu workgroups (per_cu == 0: as many as are resident per CU),
 // clamped to the tile count.
-// Frame cache of a scene: the frustum survivors an EARLY pass found (view-space sphere, objectId, LOD index and the
-// last-frame visibility it read), per tile in ascending objectId order, for the LATE pass of the same view,
-// transforms and visibility input to reuse.
+// Frame cache of a scene: the frustum survivors an EARLY pass found (view-space sphere, objectId, LOD index), per tile
+// in ascending objectId order, for the LATE pass of the same view, transforms and object range to reuse. Nothing in
+// it depends on the visibility buffer: the LATE pass reads the current visibility itself.
 struct FrameCacheDev {
     float4* sph;       // [tiles * tile] cx, cy, cz, r
-    uint2* meta;       // [tiles * tile] objectId, LOD index | last-frame visible << 31
+    uint2* meta;       // [tiles * tile] objectId, absolute LOD index
     uint32_t* counts;  // [tiles] survivors per tile
 };
 
@@ -98,7 +98,11 @@ struct InstancedJob {
 };
 hipError_t launch_instanced(const InstancedJob& job, const CompactScratch& s, uint32_t* hist, hipStream_t st);
 
-// LATE pass from a frame cache recorded with the same tile size (block x items) over the same range.
+// LATE pass from a frame cache recorded with the same tile size (block x items) over the same range. It runs over
+// super-tiles of kSuperTile objects and needs late_cached_tiles(n) counts and late_cached_entries(n) staging slots.
+constexpr uint32_t kSuperTile = 4096;
+inline uint32_t late_cached_tiles(uint32_t n) { return (uint32_t)(((uint64_t)n + kSuperTile - 1) / kSuperTile); }
+inline uint64_t late_cached_entries(uint32_t n) { return (uint64_t)late_cached_tiles(n) * kSuperTile; }
 hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
                               uint32_t items, hipStream_t st);
 // starts_dev: the jobs' tile_start values as a compact ascending array (the batched kernels stage it in LDS)

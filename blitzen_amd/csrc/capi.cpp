@@ -619,7 +619,6 @@ bz_status bz_scene_set_visibility(bz_ctx* ctx, bz_scene scene, const uint32_t* v
     Scene* s = ctx->scene(scene);
     if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
     if (n != s->n_objects || (n && !vis)) return ctx->fail(BZ_E_INVALID, "visibility size must equal n_objects");
-    s->cache.valid = false;  // the frame cache holds the last-frame visibility its EARLY pass read
     if (!n) return BZ_OK;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(s->vis, vis, (size_t)n * 4, hipMemcpyHostToDevice, st);
@@ -634,12 +633,21 @@ bz_status bz_scene_get_visibility(bz_ctx* ctx, bz_scene scene, uint32_t* vis, ui
     Scene* s = ctx->scene(scene);
     if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
     if (n != s->n_objects || (n && !vis)) return ctx->fail(BZ_E_INVALID, "visibility size must equal n_objects");
-    s->cache.valid = false;  // the frame cache holds the last-frame visibility its EARLY pass read
     if (!n) return BZ_OK;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipMemcpyAsync(vis, s->vis, (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return ctx->hip_fail(e, "visibility download");
+    return BZ_OK;
+}
+
+bz_status bz_scene_frame_cache_stats(bz_ctx* ctx, bz_scene scene, uint64_t* recorded, uint64_t* reused)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if (recorded) *recorded = s->cache.recorded;
+    if (reused) *reused = s->cache.reused;
     return BZ_OK;
 }
 
@@ -809,23 +817,36 @@ bz_status bz_pyramid_build_batch(bz_ctx* ctx, const bz_pyramid_job* jobs, uint32
 }
 
 // ---------------------------------------------------------------------------- cull
-bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
+namespace bz {
+// One bz_cull call. With `fixed` the compaction scratch is that buffer (never grown) and nothing is allocated: a
+// range that needs more, or a frame cache not yet allocated large enough, is refused.
+bz_status Ctx::cull(const bz_cull_args* a, hipStream_t st, const CompactScratch* fixed)
 {
-    if (!ctx || !a || !a->view) return BZ_E_INVALID;
-    const bz_tuning& t = ctx->tune[0];
+    const bz_tuning& t = tune[0];
     CullJob job;
-    bz_status s = ctx->make_job(a->pass, a->flags, a->scene, a->pyramid, *a->view, a->object_begin, a->object_count,
-                                a->visibility_in_dev, a->visibility_out_dev, a->cmds_dev, a->count_dev, a->capacity,
-                                t.block, t.items, &job);
+    bz_status s = make_job(a->pass, a->flags, a->scene, a->pyramid, *a->view, a->object_begin, a->object_count,
+                           a->visibility_in_dev, a->visibility_out_dev, a->cmds_dev, a->count_dev, a->capacity, t.block,
+                           t.items, &job);
     if (s) return s;
-    hipStream_t st = (hipStream_t)stream;
-    StreamScratch* sc = nullptr;
     if (job.n == 0) {  // nothing to cull: the count is 0 and the visibility buffers are untouched
         hipError_t e = launch_fill_u32(job.count, 0u, 1, st);
-        if (e != hipSuccess) return ctx->hip_fail(e, "count fill");
+        if (e != hipSuccess) return hip_fail(e, "count fill");
         return BZ_OK;
     }
-    Scene* scn = ctx->scene(a->scene);
+    // compaction scratch for `tiles` tiles of `entries` staging slots in all
+    auto compact = [&](uint32_t tiles, uint64_t entries, CompactScratch* out) -> bz_status {
+        if (fixed) {
+            if (tiles > fixed->capacity_tiles || entries > fixed->capacity_entries)
+                return fail(BZ_E_INVALID, "the range needs more compaction scratch than the fixed buffer holds");
+            *out = *fixed;
+            return BZ_OK;
+        }
+        StreamScratch* sc = nullptr;
+        bz_status r = scratch_for(st, tiles, entries, &sc);
+        if (r == BZ_OK) *out = sc->cs;
+        return r;
+    };
+    Scene* scn = scene(a->scene);
     const bool cache = (a->flags & BZ_CULL_FRAME_CACHE) != 0;
     float key[19];
     std::memcpy(key, job.view.m, sizeof job.view.m);
@@ -837,23 +858,28 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
     key[16] = job.view.znear;
     key[17] = job.view.zfar;
     Scene::FrameCache& fc = scn->cache;
+    CompactScratch cs{};
     if (cache && job.pass == PASS_LATE && fc.valid && fc.begin == job.begin && fc.n == job.n &&
-        fc.xf_version == scn->xf_version && fc.vis_in == job.vis_in &&
-        std::memcmp(fc.view_key, key, sizeof key) == 0) {
-        // the EARLY pass of this view recorded the frustum survivors: read those instead of every transform
+        fc.xf_version == scn->xf_version && std::memcmp(fc.view_key, key, sizeof key) == 0) {
+        // the EARLY pass of this view recorded the frustum survivors: read those (and the current visibility)
+        // instead of every transform
         job.n_tiles = tiles_for(job.n, fc.block, fc.items);
-        s = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * fc.block * fc.items, &sc);
+        s = compact(late_cached_tiles(job.n), late_cached_entries(job.n), &cs);
         if (s) return s;
         const FrameCacheDev d{ fc.sph, fc.meta, fc.counts };
-        hipError_t e = launch_late_cached(job, sc->cs, d, fc.block, fc.items, st);
-        if (e != hipSuccess) return ctx->hip_fail(e, "cached late-pass launch");
+        hipError_t e = launch_late_cached(job, cs, d, fc.block, fc.items, st);
+        if (e != hipSuccess) return hip_fail(e, "cached late-pass launch");
+        ++fc.reused;
         return BZ_OK;
     }
-    s = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * t.block * t.items, &sc);
+    s = compact(job.n_tiles, (uint64_t)job.n_tiles * t.block * t.items, &cs);
     if (s) return s;
     if (cache && job.pass == PASS_EARLY) {
         const uint64_t entries = (uint64_t)job.n_tiles * t.block * t.items;
         if (fc.cap_entries < entries || fc.cap_tiles < job.n_tiles) {
+            if (fixed)
+                return fail(BZ_E_STATE, "the scene's frame cache is not allocated for this range: run one bz_cull "
+                                        "with BZ_CULL_FRAME_CACHE first");
             hipError_t e = hipStreamSynchronize(st);
             for (void* p : { (void*)fc.sph, (void*)fc.meta, (void*)fc.counts })
                 if (p) (void)hipFree(p);
@@ -867,7 +893,7 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
             if (e == hipSuccess) e = hipMalloc((void**)&fc.counts, (size_t)job.n_tiles * 4);
             if (e != hipSuccess) {
                 fc.valid = false;
-                return ctx->hip_fail(e, "frame cache allocation");
+                return hip_fail(e, "frame cache allocation");
             }
             fc.cap_entries = entries;
             fc.cap_tiles = job.n_tiles;
@@ -879,18 +905,25 @@ bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
         fc.block = t.block;
         fc.items = t.items;
         fc.xf_version = scn->xf_version;
-        fc.vis_in = job.vis_in;
         const FrameCacheDev d{ fc.sph, fc.meta, fc.counts };
-        hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st, &d);
+        hipError_t e = launch_cull(job, cs, t.block, t.items, n_cu, t.grid_per_cu, st, &d);
         if (e != hipSuccess) {
             fc.valid = false;
-            return ctx->hip_fail(e, "cull launch");
+            return hip_fail(e, "cull launch");
         }
+        ++fc.recorded;
         return BZ_OK;
     }
-    hipError_t e = launch_cull(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
-    if (e != hipSuccess) return ctx->hip_fail(e, "cull launch");
+    hipError_t e = launch_cull(job, cs, t.block, t.items, n_cu, t.grid_per_cu, st);
+    if (e != hipSuccess) return hip_fail(e, "cull launch");
     return BZ_OK;
+}
+}  // namespace bz
+
+bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* a, void* stream)
+{
+    if (!ctx || !a || !a->view) return BZ_E_INVALID;
+    return ctx->cull(a, (hipStream_t)stream, nullptr);
 }
 
 // ---------------------------------------------------------------------------- LOD-instanced cull
@@ -962,6 +995,9 @@ bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* str
     if (!s->cl_sphere) return ctx->fail(BZ_E_STATE, "the scene has no cluster table (bz_scene_set_clusters)");
     if (a->cluster_tests & ~(BZ_CLUSTER_FRUSTUM | BZ_CLUSTER_HIZ | BZ_CLUSTER_CONE))
         return ctx->fail(BZ_E_INVALID, "unknown cluster test bits");
+    if (a->flags & ~BZ_CLUSTER_OBJECT_HIZ) return ctx->fail(BZ_E_INVALID, "unknown cluster cull flags");
+    if ((a->flags & BZ_CLUSTER_OBJECT_HIZ) && !a->pyramid)
+        return ctx->fail(BZ_E_INVALID, "BZ_CLUSTER_OBJECT_HIZ needs a pyramid");
     if (!a->dispatch_count_dev || !a->count_dev) return ctx->fail(BZ_E_INVALID, "NULL count buffer");
     if (a->dispatch_capacity && !a->dispatch_dev) return ctx->fail(BZ_E_INVALID, "NULL dispatch buffer");
     if ((uintptr_t)a->dispatch_dev % 4u) return ctx->fail(BZ_E_INVALID, "dispatch_dev must be 4-byte aligned");
@@ -969,10 +1005,19 @@ bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* str
         return ctx->fail(BZ_E_INVALID, "the scene can expand to more than 2^32-1 cluster records");
     const bz_tuning& t = ctx->tune[0];
     if (t.block * t.items > 2048u) return ctx->fail(BZ_E_INVALID, "object tile larger than 2048");
+    // the object pass: PreClusterDrawCull's frustum + LOD (PLAIN), or with the object Hi-Z test (TEMPORAL)
+    const bool obj_hiz = (a->flags & BZ_CLUSTER_OBJECT_HIZ) != 0;
     CullJob oj;
-    bz_status st_ = ctx->make_job(a->pyramid ? PASS_TEMPORAL : PASS_PLAIN, 0, a->scene, a->pyramid, *a->view, 0, 0,
-                                  nullptr, nullptr, a->cmds_dev, a->count_dev, a->capacity, t.block, t.items, &oj);
+    bz_status st_ = ctx->make_job(obj_hiz ? PASS_TEMPORAL : PASS_PLAIN, 0, a->scene, obj_hiz ? a->pyramid : 0,
+                                  *a->view, 0, 0, nullptr, nullptr, a->cmds_dev, a->count_dev, a->capacity, t.block,
+                                  t.items, &oj);
     if (st_) return st_;
+    PyramidDesc cl_pyr{};  // the cluster pass's Hi-Z pyramid
+    if (a->pyramid) {
+        Pyramid* p = ctx->pyramid(a->pyramid);
+        if (!p) return ctx->fail(BZ_E_INVALID, "unknown pyramid handle");
+        cl_pyr = p->desc();
+    }
     hipStream_t st = (hipStream_t)stream;
     if (oj.n == 0) {
         hipError_t e = launch_fill_u32(a->dispatch_count_dev, 0u, 1, st);
@@ -989,7 +1034,7 @@ bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* str
     std::memset(&cj, 0, sizeof cj);
     cj.scene = oj.scene;
     cj.view = oj.view;
-    cj.pyr = oj.pyr;
+    cj.pyr = cl_pyr;
     cj.cl_sphere = s->cl_sphere;
     cj.cl_cone = s->cl_cone;
     cj.cl_draw = s->cl_draw;
@@ -1000,7 +1045,7 @@ bz_status bz_cull_clusters(bz_ctx* ctx, const bz_cluster_cull_args* a, void* str
     cj.dispatch_capacity = a->dispatch_capacity;
     cj.capacity = a->capacity;
     cj.tests = a->cluster_tests;
-    cj.hiz = oj.hiz;
+    cj.hiz = a->pyramid ? 1u : 0u;
     cj.n_tiles = oj.n_tiles;
     cj.tile_size = t.block * t.items;
     hipError_t e = launch_cluster_objects(oj, sc->cs, sc->xs.ccounts, t.block, t.items, st);

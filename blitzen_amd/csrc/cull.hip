@@ -416,7 +416,7 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
             if (frus[k]) {
                 const uint32_t pos = sm.cwoff[k][wave] + crank[k];
                 fc.sph[cb + pos] = make_float4(cx[k], cy[k], cz[k], rad[k]);
-                fc.meta[cb + pos] = make_uint2(s.gi[k], lod_abs[k] | (s.vis[k] != 0u ? 0x80000000u : 0u));
+                fc.meta[cb + pos] = make_uint2(s.gi[k], lod_abs[k]);
             }
     }
     __syncthreads();  // woff / cwoff are rewritten by the next tile
@@ -704,17 +704,17 @@ __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* _
 }
 
 // LATE pass from a frame cache: the EARLY pass of the same view, transforms and range recorded each tile's frustum
-// survivors (view-space sphere, objectId, surfaceId, scale) in ascending objectId order. One workgroup per tile reads
-// the tile's last-frame visibility and its survivors only: objects outside the frustum are invisible (visibility
-// 0, no command), survivors take the Hi-Z test and, when newly visible, LOD selection -- the same arithmetic on the
-// same values as the full LATE pass, so the commands, count and visibility are identical.
-constexpr uint32_t kSuperTile = 4096;  // objects per workgroup of the cached late pass
-constexpr uint32_t kLcBlock = 512;
+// survivors (view-space sphere, objectId, LOD index) in ascending objectId order. A workgroup reads its objects'
+// current visibility (as LateDrawCull.comp.glsl:51 does, so any write to it since the EARLY pass is seen) and its
+// survivors only: objects outside the frustum are invisible (visibility 0, no command), survivors take the Hi-Z test
+// and, when newly visible, are emitted with their recorded LOD -- the same arithmetic on the same values as the full
+// LATE pass, so the commands, count and visibility are identical.
+constexpr uint32_t kLcBlock = 512;  // threads per workgroup of the cached late pass (kSuperTile objects each)
 
-// A workgroup takes a super-tile of kSuperTile objects (kSuperTile / TILE recorded tiles): the survivors of its
-// tiles, each with its LOD and the last-frame visibility the EARLY pass read, and the Hi-Z test; then the
-// visibility of the whole super-tile (coalesced; 0 outside the frustum) and the newly visible survivors' commands
-// in survivor order. Neither the visibility buffer nor the LOD tables are read.
+// A workgroup takes a super-tile of kSuperTile objects (kSuperTile / TILE recorded tiles): the visibility of its
+// objects as a bitmap in LDS, the survivors of its tiles with their LOD, and the Hi-Z test; then the visibility of
+// the whole super-tile (coalesced; 0 outside the frustum) and the newly visible survivors' commands in survivor
+// order. No transform, surface or LOD table is read.
 template <int TILE>
 __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameCacheDev fc, uint32_t n_rec_tiles,
                                                                uint32_t* counts, uint2* staging)
@@ -723,7 +723,7 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
     constexpr uint32_t S = kSuperTile / TILE;            // recorded tiles per super-tile
     constexpr uint32_t OPT = kSuperTile / BLOCK;         // objects per thread
     constexpr uint32_t U = 2;                            // survivors per thread in the first round
-    __shared__ uint32_t s_new[kSuperTile / 32];
+    __shared__ uint32_t s_new[kSuperTile / 32], s_old[kSuperTile / 32];
     __shared__ uint32_t s_lvl[kMaxMips], s_cnt[S + 1], s_woff[U][NW], s_base;
     const uint32_t st = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t base = st * kSuperTile;  // first object of the super-tile, relative to the job's range
@@ -746,6 +746,13 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
         const uint32_t t = st * S + tid;
         cnt = t < n_rec_tiles ? fc.counts[t] : 0u;
     }
+    // the super-tile's current visibility (one coalesced load per thread and item, in flight with the records)
+    uint32_t old[OPT];
+#pragma unroll
+    for (uint32_t k = 0; k < OPT; ++k) {
+        const uint32_t i = k * BLOCK + tid;
+        old[k] = i < nt ? J.vis_in[obj0 + i] : 0u;
+    }
     if (wave == 0) {  // prefix of the recorded tiles' counts (S <= 8 lanes)
         uint32_t x = lane < S ? cnt : 0u;
 #pragma unroll
@@ -763,6 +770,15 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
     }
     for (uint32_t x = tid; x < kSuperTile / 32; x += BLOCK) s_new[x] = 0u;
     if (tid == 0) s_base = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < OPT; ++k) {  // objects k * BLOCK + wave * 64 + lane: two bitmap words per wave and item
+        const unsigned long long m = __ballot(old[k] != 0u);
+        if (lane == 0) {
+            const uint32_t w = (k * BLOCK + wave * 64u) >> 5;
+            s_old[w] = (uint32_t)m;
+            s_old[w + 1] = (uint32_t)(m >> 32);
+        }
+    }
     __syncthreads();
     const uint32_t total = s_cnt[S];
     uint2* stage = staging + (size_t)st * kSuperTile;
@@ -784,8 +800,8 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
                 const bool vis =
                     test ? occlusion_visible(J.view, J.pyr, s_lvl, a0, a1, a2, a3, sp[k].z, sp[k].w) : true;
                 if (vis) atomicOr(&s_new[local >> 5], 1u << (local & 31u));
-                emit[k] = vis && (mt[k].y & 0x80000000u) == 0u;
-                lod_abs[k] = mt[k].y & 0x7fffffffu;
+                emit[k] = vis && ((s_old[local >> 5] >> (local & 31u)) & 1u) == 0u;
+                lod_abs[k] = mt[k].y;
             }
         }
         uint32_t rank[U];
@@ -970,7 +986,7 @@ hipError_t launch_late_cached(const CullJob& job_in, const CompactScratch& s, co
     CullJob job = job_in;
     const uint32_t rec = block * items;
     const uint32_t n_rec_tiles = (job.n + rec - 1) / rec;
-    job.n_tiles = (job.n + kSuperTile - 1) / kSuperTile;
+    job.n_tiles = late_cached_tiles(job.n);
     switch (rec) {
     case 512: return launch_late_cached_t<512>(job, s, cache, n_rec_tiles, st);
     case 1024: return launch_late_cached_t<1024>(job, s, cache, n_rec_tiles, st);

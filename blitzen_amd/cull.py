@@ -163,6 +163,13 @@ class Context:
         self.check(lib().bz_scene_visibility_ptr(self.handle, scene, C.byref(p)), "bz_scene_visibility_ptr")
         return p.value
 
+    def frame_cache_stats(self, scene: int) -> tuple[int, int]:
+        """(EARLY passes that recorded survivors, LATE passes that read them) for the scene."""
+        rec, reu = C.c_uint64(), C.c_uint64()
+        self.check(lib().bz_scene_frame_cache_stats(self.handle, scene, C.byref(rec), C.byref(reu)),
+                   "bz_scene_frame_cache_stats")
+        return rec.value, reu.value
+
     def update_transforms(self, scene: int, ids: np.ndarray, transforms: np.ndarray, stream=None):
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
         transforms = np.ascontiguousarray(transforms, dtype=MESH_TRANSFORM)
@@ -217,12 +224,13 @@ class Context:
 
     def cull_clusters(self, scene: int, view: CameraViewData, tests: int, dispatch: DeviceBuffer,
                       dispatch_count: DeviceBuffer, cmds: DeviceBuffer, count: DeviceBuffer, pyramid: int = 0,
-                      dispatch_capacity: int | None = None, capacity: int | None = None, stream=None):
+                      dispatch_capacity: int | None = None, capacity: int | None = None, flags: int = 0,
+                      stream=None):
         if dispatch_capacity is None:
             dispatch_capacity = dispatch.nbytes // 12
         if capacity is None:
             capacity = cmds.nbytes // 24
-        a = A.ClusterCullArgs(scene=scene, pyramid=pyramid, cluster_tests=tests, flags=0, view=C.pointer(view),
+        a = A.ClusterCullArgs(scene=scene, pyramid=pyramid, cluster_tests=tests, flags=flags, view=C.pointer(view),
                               dispatch_dev=dispatch.ptr, dispatch_count_dev=dispatch_count.ptr,
                               dispatch_capacity=dispatch_capacity, cmds_dev=cmds.ptr, count_dev=count.ptr,
                               capacity=capacity)

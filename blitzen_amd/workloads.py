@@ -194,7 +194,8 @@ CLUSTER_SEED = 0xB1172E04
 class ClusterFrames:
     """Cluster-mode workload: 1 048 576 objects over 128 surfaces with 16-128 clusters of 64 triangles each (cone
     axis uniform on the sphere, cutoff U[-0.5, 0.9]); per frame 1% of the transforms get seeded rotation updates,
-    then the two-level cull (object pass, expansion, cluster frustum + Hi-Z + cone tests) runs against a 1920x1080
+    then the two-level cull runs: the object pass of PreClusterDrawCull.comp.glsl:13-47 (frustum + LOD, no Hi-Z),
+    the expansion of the chosen LODs' clusters, and the cluster frustum + Hi-Z + cone tests against a 1920x1080
     depth buffer's 1024x512 11-mip pyramid."""
     ctx: bz.Context
     scene: int

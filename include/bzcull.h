@@ -229,14 +229,16 @@ BZ_API void bz_pyramid_reference_extent(uint32_t w, uint32_t h, uint32_t* base_w
 
 /* Flags */
 #define BZ_CULL_EARLY_HIZ 0x1u /* EARLY pass also tests against `pyramid` (the previous frame's pyramid) */
-/* Frame cache. EARLY: also runs the frustum test on every object of the range (last-frame visible or not) and keeps
- * the survivors' view-space spheres and last-frame visibility in the scene. LATE: when the scene holds survivors
- * recorded with the same view matrix, frustum, object range and visibility input, and no transform update or
- * bz_scene_set_visibility since, reads only those (not every object's transform and visibility) and writes the
- * visibility of the range; otherwise runs in full. Outputs are identical either way. As in the reference frame,
- * the visibility input is not modified between the two passes, and both calls go on the same stream (or the
- * caller orders them). */
+/* Frame cache. EARLY: also runs the frustum test (and LOD selection) on every object of the range, last-frame
+ * visible or not, and keeps the survivors' view-space spheres and LODs in the scene. LATE: when the scene holds
+ * survivors recorded with the same view matrix, frustum constants, lodTarget and object range, and no transform
+ * update since, reads those and the current visibility input (not every object's transform), and writes the
+ * visibility of the range; otherwise runs in full. Outputs are identical either way, whatever was written to any
+ * visibility buffer in between (by other passes, bz_scene_set_visibility or the caller's own kernels). Both calls go
+ * on the same stream, or the caller orders them. */
 #define BZ_CULL_FRAME_CACHE 0x2u
+/* Frame-cache use of a scene: EARLY passes that recorded survivors, and LATE passes that read them. */
+BZ_API bz_status bz_scene_frame_cache_stats(bz_ctx* ctx, bz_scene scene, uint64_t* recorded, uint64_t* reused);
 
 typedef struct bz_cull_args {
     bz_scene scene;
@@ -260,17 +262,20 @@ typedef struct bz_cull_args {
 BZ_API bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* args, void* stream);
 
 /* Cluster two-level cull (PreClusterDrawCull.comp.glsl:13-47 then InitialClusterCull.comp.glsl:12-55).
- * Object pass: frustum (+ Hi-Z if a pyramid is given) + LOD; expansion of the chosen LOD's clusters in
- * ascending (objectId, clusterId) order; cluster pass: per-cluster frustum, Hi-Z and backface-cone tests
- * (cluster tests are this library's definition; the reference leaves them commented out). */
+ * Object pass: frustum + LOD, as PreClusterDrawCull (with BZ_CLUSTER_OBJECT_HIZ in `flags`, also the Hi-Z test
+ * against `pyramid`, which the reference's object pass does not have); expansion of the chosen LOD's clusters in
+ * ascending (objectId, clusterId) order; cluster pass: per-cluster frustum, Hi-Z (against `pyramid`) and
+ * backface-cone tests, as selected by `cluster_tests` (cluster tests are this library's definition; the reference
+ * leaves them commented out). */
 #define BZ_CLUSTER_FRUSTUM 0x1u
 #define BZ_CLUSTER_HIZ     0x2u
 #define BZ_CLUSTER_CONE    0x4u
+#define BZ_CLUSTER_OBJECT_HIZ 0x1u /* bz_cluster_cull_args.flags */
 typedef struct bz_cluster_cull_args {
     bz_scene scene;
     bz_pyramid pyramid;                       /* 0 = no Hi-Z test at either level */
-    uint32_t cluster_tests;                   /* BZ_CLUSTER_* */
-    uint32_t flags;                           /* reserved, 0 */
+    uint32_t cluster_tests;                   /* BZ_CLUSTER_FRUSTUM | BZ_CLUSTER_HIZ | BZ_CLUSTER_CONE */
+    uint32_t flags;                           /* 0 or BZ_CLUSTER_OBJECT_HIZ */
     const bz_CameraViewData* view;
     bz_ClusterDispatchData* dispatch_dev;     /* expansion output (the reference's ClusterDispatchBuffer) */
     uint32_t* dispatch_count_dev;             /* number of expanded clusters */

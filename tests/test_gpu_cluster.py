@@ -44,14 +44,15 @@ def pyramid(ctx):
     buf.free()
 
 
-def run_gpu(ctx, h, v, tests, pyr=0, dcap=1 << 21, cap=1 << 21):
+def run_gpu(ctx, h, v, tests, pyr=0, dcap=1 << 21, cap=1 << 21, flags=0):
     disp = ctx.alloc(max(dcap, 1) * 12)
     dcnt = ctx.alloc(4)
     cmds = ctx.alloc(max(cap, 1) * 24)
     cnt = ctx.alloc(4)
     dcnt.fill_u32(0xDEADBEEF, 1)
     cnt.fill_u32(0xDEADBEEF, 1)
-    ctx.cull_clusters(h, v, tests, disp, dcnt, cmds, cnt, pyramid=pyr, dispatch_capacity=dcap, capacity=cap)
+    ctx.cull_clusters(h, v, tests, disp, dcnt, cmds, cnt, pyramid=pyr, dispatch_capacity=dcap, capacity=cap,
+                      flags=flags)
     ctx.check_kernels()
     nd = int(dcnt.download(np.uint32, 1)[0])
     recs = disp.download(A.CLUSTER_DISPATCH, min(nd, dcap)) if min(nd, dcap) else np.zeros(0, A.CLUSTER_DISPATCH)
@@ -61,12 +62,14 @@ def run_gpu(ctx, h, v, tests, pyr=0, dcap=1 << 21, cap=1 << 21):
     return np.ascontiguousarray(recs).view(np.uint32).reshape(-1, 3), nd, H.cmds_u32(out), n
 
 
-def run_oracle(sc, v, tests, opyr=None, dcap=1 << 21, cap=1 << 21, transforms=None, objects=None):
+def run_oracle(sc, v, tests, opyr=None, dcap=1 << 21, cap=1 << 21, transforms=None, objects=None, object_hiz=False):
+    """The object pass (PreClusterDrawCull.comp.glsl:13-47: frustum + LOD; with object_hiz also the Hi-Z test
+    against opyr), the expansion, then the cluster tests (Hi-Z against opyr when given)."""
     objects = sc.objects if objects is None else objects
     transforms = sc.transforms if transforms is None else transforms
     scene = (objects, transforms, sc.surfaces, sc.lods)
     vb = H.view_bytes(v)
-    recs, nd = orc.cluster_expand(scene, vb, opyr, capacity=dcap)
+    recs, nd = orc.cluster_expand(scene, vb, opyr if object_hiz else None, capacity=dcap)
     cmds, n = orc.cluster_cull(recs, objects, transforms, sc.clusters, vb, opyr, tests, capacity=cap)
     return recs, nd, cmds, n
 
@@ -80,16 +83,26 @@ def check(got, exp):
 
 
 @pytest.mark.parametrize("tests", [0, A.CLUSTER_FRUSTUM, A.CLUSTER_CONE, A.CLUSTER_FRUSTUM | A.CLUSTER_CONE, ALL])
-@pytest.mark.parametrize("with_pyr", [False, True])
-def test_cluster_cull_bit_exact(ctx, cl_scene, pyramid, tests, with_pyr):
+@pytest.mark.parametrize("with_pyr,object_hiz", [(False, False), (True, False), (True, True)])
+def test_cluster_cull_bit_exact(ctx, cl_scene, pyramid, tests, with_pyr, object_hiz):
     h, sc = cl_scene
     v = H.view(pyr=(1024, 512))
     v.lodTarget = 2e-5  # finer LODs: more clusters per object
     p, opyr = pyramid
-    got = run_gpu(ctx, h, v, tests, pyr=p if with_pyr else 0)
-    exp = run_oracle(sc, v, tests, opyr if with_pyr else None)
+    got = run_gpu(ctx, h, v, tests, pyr=p if with_pyr else 0, flags=A.CLUSTER_OBJECT_HIZ if object_hiz else 0)
+    exp = run_oracle(sc, v, tests, opyr if with_pyr else None, object_hiz=object_hiz)
     assert exp[1] > 10000
     check(got, exp)
+
+
+def test_cluster_object_hiz_flag_checked(ctx, cl_scene):
+    h, sc = cl_scene
+    b = ctx.alloc(64)
+    with pytest.raises(bz.BzError):  # the object Hi-Z test needs a pyramid
+        ctx.cull_clusters(h, H.view(), 0, b, b, b, b, dispatch_capacity=1, capacity=1, flags=A.CLUSTER_OBJECT_HIZ)
+    with pytest.raises(bz.BzError):  # unknown flag bits
+        ctx.cull_clusters(h, H.view(), 0, b, b, b, b, dispatch_capacity=1, capacity=1, flags=0x10)
+    b.free()
 
 
 def test_cluster_capacities(ctx, cl_scene, pyramid):
@@ -98,6 +111,7 @@ def test_cluster_capacities(ctx, cl_scene, pyramid):
     v.lodTarget = 2e-5
     p, opyr = pyramid
     full = run_oracle(sc, v, ALL, opyr)
+    assert full[1] > 20000
     for dcap, cap in [(full[1] // 2, 1 << 21), (1 << 21, 100), (1000, 10), (0, 0), (1, 1)]:
         check(run_gpu(ctx, h, v, ALL, pyr=p, dcap=dcap, cap=cap), run_oracle(sc, v, ALL, opyr, dcap=dcap, cap=cap))
 

@@ -159,35 +159,112 @@ def test_frame_cache_not_used_for_another_view_or_range(ctx, pyramid):
     ctx.scene_destroy(h2)
 
 
-def test_frame_cache_keyed_on_visibility_input(ctx, pyramid):
-    """A late pass reading another visibility buffer than the recording early pass, or after the scene's visibility
-    was replaced, runs in full and still matches the oracle."""
+def late_exp(scene, v, opyr, vis):
+    """The oracle's LATE pass reading and rewriting `vis` in place (LateDrawCull.comp.glsl:51,71)."""
+    return orc.cull(scene, H.view_bytes(v), A.PASS_LATE, opyr, vis_in=vis, vis_out=vis)
+
+
+def check(got, exp):
+    assert got[1] == exp[1], (got[1], exp[1])
+    np.testing.assert_array_equal(got[0], exp[0])
+
+
+def test_frame_cache_reads_current_visibility(ctx, pyramid):
+    """The cached LATE pass reads the visibility it is given, not what the recording EARLY pass read: another buffer,
+    or the scene's buffer replaced in between, gives the oracle's result and still uses the records."""
     n = 50000
-    sc = H.c1_scene(n)
-    scene = H.as_tuple(sc)
+    scene = H.as_tuple(H.c1_scene(n))
     h = ctx.scene_create(*scene)
     v = H.view(pyr=(1024, 512))
     p, opyr = pyramid
     vis_a = bz.synth_visibility(9, n, 0.5)
     vis_b = bz.synth_visibility(10, n, 0.5)
     ctx.set_visibility(h, vis_a)
-    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)  # records with the scene's buffer (vis_a)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)  # records, reading the scene's buffer (vis_a)
     other = ctx.alloc(n * 4)
     other.upload(vis_b)
     got = run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC, vis_in=other.ptr)  # reads vis_b in place
     vexp = vis_b.copy()
-    exp = orc.cull(scene, H.view_bytes(v), A.PASS_LATE, opyr, vis_in=vis_b, vis_out=vexp)
-    assert got[1] == exp[1]
-    np.testing.assert_array_equal(got[0], exp[0])
+    check(got, late_exp(scene, v, opyr, vexp))
     np.testing.assert_array_equal(other.download(np.uint32, n), vexp)
-    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)  # records vis_a again
-    ctx.set_visibility(h, vis_b)  # replaced: the recorded bits no longer apply
+    assert ctx.frame_cache_stats(h) == (1, 1)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)
+    ctx.set_visibility(h, vis_b)  # replaced between the passes
     got = run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC)
     vexp = vis_b.copy()
-    exp = orc.cull(scene, H.view_bytes(v), A.PASS_LATE, opyr, vis_in=vis_b, vis_out=vexp)
-    np.testing.assert_array_equal(got[0], exp[0])
+    check(got, late_exp(scene, v, opyr, vexp))
     np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
+    assert ctx.frame_cache_stats(h) == (2, 2)
     other.free()
+    ctx.scene_destroy(h)
+
+
+def test_frame_cache_late_after_late_in_place(ctx, pyramid):
+    """EARLY(cache) -> LATE(cache, in place) -> LATE(cache): the second LATE reads the visibility the first one wrote,
+    as the reference's LATE pass would, so it emits only what is newly visible relative to that."""
+    n = 70000
+    scene = H.as_tuple(H.c1_scene(n))
+    h = ctx.scene_create(*scene)
+    v = H.view(pyr=(1024, 512))
+    p, opyr = pyramid
+    vis0 = bz.synth_visibility(21, n, 0.5)
+    ctx.set_visibility(h, vis0)
+    check(run(ctx, h, v, A.PASS_EARLY, n, flags=FC),
+          orc.cull(scene, H.view_bytes(v), A.PASS_EARLY, vis_in=vis0))
+    vexp = vis0.copy()
+    first = late_exp(scene, v, opyr, vexp)
+    check(run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC), first)
+    second = late_exp(scene, v, opyr, vexp)
+    assert first[1] > 0 and second[1] == 0  # everything visible now was visible before the second pass
+    check(run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC), second)
+    np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
+    # an uncached LATE pass in place between them reads and writes the same buffer: the cached one after it agrees
+    ctx.set_visibility(h, vis0)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)
+    vexp = vis0.copy()
+    check(run(ctx, h, v, A.PASS_LATE, n, pyr=p), late_exp(scene, v, opyr, vexp))
+    check(run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC), late_exp(scene, v, opyr, vexp))
+    np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
+    assert ctx.frame_cache_stats(h) == (2, 3)
+    ctx.scene_destroy(h)
+
+
+def test_frame_cache_after_device_visibility_writes(ctx, pyramid):
+    """Writes to the scene's visibility through its device pointer (taken before the EARLY pass) or by a batched LATE
+    job between EARLY(cache) and LATE(cache) are seen by the cached LATE pass."""
+    n = 65000
+    scene = H.as_tuple(H.c1_scene(n))
+    h = ctx.scene_create(*scene)
+    v = H.view(pyr=(1024, 512))
+    p, opyr = pyramid
+    vptr = ctx.visibility_ptr(h)
+    vis0 = bz.synth_visibility(31, n, 0.5)
+    ctx.set_visibility(h, vis0)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)
+    # the caller's own writes: a range set to 0, another to 1, through the pointer handed out before the EARLY pass
+    lib = bz.lib()
+    ctx.check(lib.bz_memset_d32(ctx.handle, vptr + 4 * 1000, 0, 20000, None), "memset")
+    ctx.check(lib.bz_memset_d32(ctx.handle, vptr + 4 * 30000, 1, 20000, None), "memset")
+    vexp = vis0.copy()
+    vexp[1000:21000] = 0
+    vexp[30000:50000] = 1
+    check(run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC), late_exp(scene, v, opyr, vexp))
+    np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
+    # a batched LATE job rewriting the scene's visibility in place between the passes
+    vis1 = bz.synth_visibility(32, n, 0.3)
+    ctx.set_visibility(h, vis1)
+    run(ctx, h, v, A.PASS_EARLY, n, flags=FC)
+    bcmds, bcnt = ctx.alloc(n * 24), ctx.alloc(4)
+    v2 = H.view(pyr=(1024, 512), yaw=0.3)
+    ctx.cull_batch([ctx.view_job(h, v2, A.PASS_LATE, bcmds, bcnt, pyramid=p)])
+    ctx.check_kernels()
+    vexp = vis1.copy()
+    late_exp(scene, v2, opyr, vexp)
+    check(run(ctx, h, v, A.PASS_LATE, n, pyr=p, flags=FC), late_exp(scene, v, opyr, vexp))
+    np.testing.assert_array_equal(ctx.get_visibility(h, n), vexp)
+    assert ctx.frame_cache_stats(h) == (2, 2)
+    bcmds.free()
+    bcnt.free()
     ctx.scene_destroy(h)
 
 
