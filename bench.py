@@ -52,29 +52,29 @@ def ensure_built():
         __graft_entry__.build()
 
 
-def cpu_baseline(scene, vb, depth, vis0, budget_s: float):
-    """The oracle (OpenMP port of the reference shaders) on the host cores, full C1 frame, bounded by budget_s."""
-    from oracle import pyoracle as orc
+def cpu_baseline(budget_s: float):
+    """The oracle (OpenMP port of the reference shaders) on the host's cores, full C1 frame: one thread per physical
+    core available to this process (OMP_PROC_BIND=close, OMP_PLACES=cores) and one thread, each the median of >= 10
+    runs after 2 warm-ups, in a child process (oracle/cpu_frame.py) so that the binding applies to the OpenMP runtime
+    there and not to this GPU process."""
+    import subprocess
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    runs, times = 0, []
-    t_start = time.perf_counter()
-    while True:
-        t0 = time.perf_counter()
-        texels = orc.pyramid_build(depth, PYR_W, PYR_H, PYR_MIPS, threads=threads)
-        pyr = orc.pyramid(texels, PYR_W, PYR_H, PYR_MIPS)
-        orc.cull(scene, vb, 0, vis_in=vis0, threads=threads)
-        vout = vis0.copy()
-        orc.cull(scene, vb, 1, pyr=pyr, vis_in=vis0, vis_out=vout, threads=threads)
-        times.append(time.perf_counter() - t0)
-        runs += 1
-        if runs >= 3 and (time.perf_counter() - t_start > budget_s or runs >= 30):
-            break
-    med = statistics.median(times[1:]) if len(times) > 1 else times[0]
-    return {"value": N_OBJECTS / med, "unit": "object-culls/s", "cores": threads, "kind": "port",
-            "sample": f"full frame (pyramid + early + late, {N_OBJECTS} objects), median of {len(times) - 1} runs "
-                      f"after 1 warm-up, OpenMP {threads} threads",
-            "ms_per_frame": med * 1e3}
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_frame", "--seconds", str(budget_s)], cwd=str(ROOT),
+                       env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    m, one = d["all_cores"], d["single_thread"]
+    return {"value": m["object_culls_per_s"], "unit": "object-culls/s", "cores": m["threads"], "kind": "port",
+            "sample": f"full C1 frame (pyramid + early + late, {N_OBJECTS} objects), median of {m['runs']} runs after "
+                      f"2 warm-ups, OpenMP {m['threads']} threads on {m['threads']} physical cores "
+                      f"(OMP_PROC_BIND=close OMP_PLACES=cores), {d['cpu_model']}",
+            "ms_per_frame": m["ms_per_frame"],
+            "single_thread": {"value": one["object_culls_per_s"], "ms_per_frame": one["ms_per_frame"],
+                              "runs": one["runs"]},
+            "cpu_model": d["cpu_model"], "nproc": d["nproc"], "physical_cores_available": d["physical_cores_available"],
+            "cgroup_cpu_quota": d["cgroup_cpu_quota"], "omp_num_threads_env": d["omp_num_threads_env"]}
 
 
 def pct(xs, q):
@@ -149,7 +149,10 @@ def bench_batched(ctx, stream, steps: int, warmup: int):
 
 
 def bench_served(ctx, rates, duration_s: float):
-    """Poisson request streams against the serving loop over 8 resident scenes; latency = completion - submission."""
+    """Poisson request streams against the serving loop over 8 resident scenes. Latency is measured from each
+    request's scheduled arrival to its completion, so a submitter that falls behind the schedule shows up in it
+    (no coordinated omission); the submitter's own delay (submission - arrival) and the latency from submission are
+    reported beside it."""
     import numpy as np
 
     from blitzen_amd import _capi as A
@@ -165,7 +168,7 @@ def bench_served(ctx, rates, duration_s: float):
                        f"{W.SERVE_DEPTH_SIZES[0]}..{W.SERVE_DEPTH_SIZES[-1]} rasterised on the device from its seed "
                        f"({W.SERVE_OCCLUDERS} occluders), own pyramid, Hi-Z cull of the whole scene; 64 GB HBM "
                        f"budget; 4 batches in flight, <= 64 requests per batch",
-           "rates": []}
+           "latency": "scheduled arrival -> completion (submit_delay: arrival -> submission)", "rates": []}
     try:
         warm, offs, _ = W.served_requests(scenes, 2000, 0.02, seed=1)
         tk, _ = ctx.submit_schedule(warm, offs)
@@ -175,28 +178,34 @@ def bench_served(ctx, rates, duration_s: float):
         for rate in rates:
             reqs, offs, objs = W.served_requests(scenes, rate, duration_s)
             tickets, st = ctx.submit_schedule(reqs, offs)
-            lat, sub, comp, done_objs, bsz, bad = [], [], [], 0, [], 0
+            lat, lat_sub, delay, arr, comp, done_objs, bsz, bad = [], [], [], [], [], 0, [], 0
             for t, n_obj in zip(tickets, objs):
                 if not t:
                     bad += 1
                     continue
                 r = ctx.wait(int(t), 120)
                 if r.status == A.OK:
-                    lat.append((r.complete_ns - r.submit_ns) / 1e6)
-                    sub.append(r.submit_ns)
+                    lat.append((r.complete_ns - r.arrival_ns) / 1e6)
+                    lat_sub.append((r.complete_ns - r.submit_ns) / 1e6)
+                    delay.append((r.submit_ns - r.arrival_ns) / 1e6)
+                    arr.append(r.arrival_ns)
                     comp.append(r.complete_ns)
                     done_objs += int(n_obj)
                     bsz.append(r.batch_size)
                 else:
                     bad += 1
                 ctx.release(int(t))
-            span = (max(comp) - min(sub)) / 1e9 if comp else float("nan")
+            span = (max(comp) - min(arr)) / 1e9 if comp else float("nan")
             out["rates"].append({
                 "offered_req_per_s": rate, "requests": len(reqs), "failed": bad,
                 "achieved_req_per_s": round(len(lat) / span, 1),
                 "object_culls_per_s": round(done_objs / span, 1),
                 "latency_ms_p50": round(pct(lat, 50), 4), "latency_ms_p99": round(pct(lat, 99), 4),
-                "latency_ms_max": round(max(lat), 4), "mean_batch": round(statistics.mean(bsz), 2)})
+                "latency_ms_max": round(max(lat), 4),
+                "submit_delay_ms_p50": round(pct(delay, 50), 4), "submit_delay_ms_p99": round(pct(delay, 99), 4),
+                "latency_from_submit_ms_p50": round(pct(lat_sub, 50), 4),
+                "latency_from_submit_ms_p99": round(pct(lat_sub, 99), 4),
+                "mean_batch": round(statistics.mean(bsz), 2)})
             log(f"bench: served {rate}/s -> p50 {out['rates'][-1]['latency_ms_p50']} ms, "
                 f"p99 {out['rates'][-1]['latency_ms_p99']} ms, {out['rates'][-1]['achieved_req_per_s']} req/s")
         st = ctx.server_stats()
@@ -341,7 +350,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--copies", type=int, default=4)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
@@ -463,15 +472,25 @@ def main():
     # roofline unit: the whole frame. With the frame cache the EARLY launch does part of the LATE pass's work
     # (every object's frustum test), so per-pass figures no longer divide a pass's own bytes by its own time.
     achieved = frame_bytes / (frame_ms * 1e-3) / 1e9
+    achieved_wall = frame_bytes / (ms_per_step * 1e-3) / 1e9
     traffic = load_traffic()
+    from blitzen_amd.build import kernel_sources_digest
+
+    sources = kernel_sources_digest()
+    traffic_ok = bool(traffic) and not args.no_frame_cache and traffic.get("kernel_sources") == sources
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get("bytes_per_launch") if traffic and not args.no_frame_cache else None,
+                "achieved_wall": round(achieved_wall, 1), "frac_wall": round(achieved_wall / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get("bytes_per_launch") if traffic_ok else None,
+                "kernel_sources": sources,
                 "kernel": ("C1 frame: pyramid_kernel + decide_kernel (EARLY) + emit_kernel + decide_kernel (LATE) + "
                            "emit_kernel" if args.no_frame_cache else
                            "C1 frame: pyramid_kernel + decide_kernel (EARLY, frame cache) + emit_kernel + "
                            "late_cached_kernel + emit_kernel"),
                 "algorithmic_bytes": frame_bytes, "avg_us": round(frame_ms * 1e3, 2)}
+    if traffic and not traffic_ok and not args.no_frame_cache:  # measured on other kernels: reported, not used
+        roofline["traffic_stale"] = {"bytes_per_launch": traffic.get("bytes_per_launch"),
+                                     "measured_sources": traffic.get("kernel_sources")}
 
     result = {
         "metric": "object-culls/sec (4M objs, 1 view) + batched views/sec, p50/p99 latency",
@@ -515,8 +534,7 @@ def main():
                                               max(5, min(args.steps, 20)))
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
-        result["cpu_baseline"] = cpu_baseline((sc.objects, sc.transforms, sc.surfaces, sc.lods), bytes(memoryview(v)),
-                                              depth, vis0, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     ctx.close()
     print(json.dumps(result), flush=True)
 

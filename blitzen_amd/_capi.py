@@ -98,7 +98,7 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("count", C.c_uint32), ("submit_ns", C.c_uint64), ("admit_ns", C.c_uint64),
                 ("complete_ns", C.c_uint64), ("batch_id", C.c_uint64), ("batch_size", C.c_uint32),
-                ("reserved", C.c_uint32), ("user_tag", C.c_uint64)]
+                ("reserved", C.c_uint32), ("user_tag", C.c_uint64), ("arrival_ns", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
@@ -146,6 +146,7 @@ SIGNATURES = {
     "bz_scene_get_visibility": (_I32, [_VP, _U64, _VP, _U32, _VP]),
     "bz_scene_visibility_ptr": (_I32, [_VP, _U64, C.POINTER(_VP)]),
     "bz_scene_frame_cache_stats": (_I32, [_VP, _U64, C.POINTER(_U64), C.POINTER(_U64)]),
+    "bz_scene_get_transforms": (_I32, [_VP, _U64, _U32, _U32, _VP, _VP]),
     "bz_scene_info": (_I32, [_VP, _U64, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32),
                              C.POINTER(_U64)]),
     "bz_pyramid_create": (_I32, [_VP, _U32, _U32, _U32, C.POINTER(_U64)]),

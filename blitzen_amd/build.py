@@ -36,6 +36,20 @@ HIPCC_FLAGS = [
 ]
 
 
+def kernel_sources_digest() -> str:
+    """SHA-256 (first 16 hex digits) of the library's sources and public header: profiles record it so that a
+    measurement can be told apart from one of other kernels."""
+    import hashlib
+
+    h = hashlib.sha256()
+    files = sorted(p for p in CSRC.iterdir() if p.suffix in (".hip", ".cpp", ".h")) + [PKG.parent / "include" /
+                                                                                         "bzcull.h"]
+    for p in files:
+        h.update(p.name.encode())
+        h.update(p.read_bytes())
+    return h.hexdigest()[:16]
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():

@@ -35,6 +35,7 @@ struct Scene {
     uint2* cl_draw = nullptr;
     uint64_t bytes = 0;
     uint64_t xf_version = 0;            // bumped by every transform update (invalidates the frame cache)
+    uint32_t* xf_winner = nullptr;      // [n_transforms] last-occurrence resolution of a commit's ids, zero between
     // frame cache: the frustum survivors of the last EARLY pass that recorded one (BZ_CULL_FRAME_CACHE); valid for
     // the same view, object range and transforms (the visibility is read by the LATE pass itself)
     struct FrameCache {
@@ -53,6 +54,7 @@ struct Scene {
     // [ids | 32-byte MeshTransforms], used in turn
     struct XfSlot {
         uint8_t* host = nullptr;       // pinned, read by the scatter kernel: cap ids, then cap transforms at xf_offset
+        uint32_t* dev_ids = nullptr;   // [cap] device copy of the committed ids
         uint32_t cap = 0;
         size_t xf_offset = 0;
         hipEvent_t done = nullptr;     // the last commit from this slot completed

@@ -43,13 +43,24 @@ struct CullJob {
 };
 
 // Scratch of the two-kernel compaction: per-tile command counts and per-tile staging slots (objectId, LOD) of
-// tile_size entries each. Fully rewritten by every decide launch; nothing needs clearing between launches.
+// tile_size entries each, and room for the counts' exclusive prefix. Fully rewritten by every decide launch; nothing
+// needs clearing between launches. `counts` and `prefix` share one allocation of counts_words(capacity_tiles).
 struct CompactScratch {
     uint32_t* counts;            // [capacity_tiles]
     uint2* staging;              // [capacity_entries]
     uint32_t capacity_tiles;
     uint64_t capacity_entries;
+    uint32_t* prefix;            // [capacity_tiles + 1] exclusive prefix of counts (written by the scan, when used)
 };
+constexpr uint64_t counts_words(uint32_t capacity_tiles) { return 2ull * capacity_tiles + 1u; }
+inline void set_prefix(CompactScratch& s) { s.prefix = s.counts ? s.counts + s.capacity_tiles : nullptr; }
+
+// An emit workgroup adds the counts of its job's earlier tiles itself when there are at most kScanTiles of them (one
+// round of loads). A job with more tiles gets their exclusive prefix from one scan launch between the decide and the
+// emit instead, so the emit's work per workgroup stays constant and the pass stays linear in the object count.
+constexpr uint32_t kScanTiles = 4096;
+// Exclusive prefix of counts[0, n) into prefix[0, n], prefix[n] = the total (one workgroup, linear in n).
+hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* prefix, hipStream_t st);
 
 struct PyrBuildJob {
     const float* depth;
@@ -106,8 +117,16 @@ inline uint64_t late_cached_entries(uint32_t n) { return (uint64_t)late_cached_t
 hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
                               uint32_t items, hipStream_t st);
 // starts_dev: the jobs' tile_start values as a compact ascending array (the batched kernels stage it in LDS)
+// max_job_tiles: the largest job's tile count (above kScanTiles the emit reads a scanned prefix of the batch)
 hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs, uint32_t total_tiles,
-                             const CompactScratch& s, uint32_t block, uint32_t items, hipStream_t st);
+                             uint32_t max_job_tiles, const CompactScratch& s, uint32_t block, uint32_t items,
+                             hipStream_t st);
+inline uint32_t max_job_tiles(const CullJob* jobs, size_t n)
+{
+    uint32_t m = 0;
+    for (size_t k = 0; k < n; ++k) m = jobs[k].n_tiles > m ? jobs[k].n_tiles : m;
+    return m;
+}
 hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st);
 // starts_dev: the jobs' wg_start values as a compact ascending array (the batched kernels stage it in LDS)
 hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
@@ -116,9 +135,11 @@ hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, floa
                                 hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t st);
 // Staged updates: ids[i] and the 32-byte MeshTransform record i (pos.xyz, scale | quaternion) scattered into the SoA
-// streams; an id >= n_xf is skipped and sets err[2].
+// streams, the last occurrence of an id winning; an id >= n_xf is skipped and sets err[2]. dev_ids: n device words
+// (a copy of the ids); winner: n_xf device words, all zero before and after the launches.
 hipError_t launch_scatter_staged_transforms(const uint32_t* ids, const float4* records, uint32_t n, uint32_t n_xf,
-                                            float4* xf_ps, float4* xf_q, uint32_t* err, hipStream_t st);
+                                            uint32_t* dev_ids, uint32_t* winner, float4* xf_ps, float4* xf_q,
+                                            uint32_t* err, hipStream_t st);
 hipError_t launch_synth_depth(uint64_t seed, uint32_t w, uint32_t h, uint32_t occluders, float p11, float znear,
                               float* depth, hipStream_t st);
 
@@ -153,14 +174,16 @@ struct ClusterJob {
     uint32_t dispatch_capacity, capacity;
     uint32_t tests, hiz;        // BZ_CLUSTER_* and whether a pyramid is bound
     uint32_t n_tiles, tile_size;
+    uint32_t scanned;           // more than kScanTiles tiles: record bases and survivor prefixes come from scans
 };
 
 // Scratch of the cluster pass: per object tile the cluster total (written by the object pass), the first record
 // and the survivor count; survivors (objectId, clusterId) compacted inside each tile's record range.
 struct ClusterScratch {
     uint32_t* ccounts;          // [capacity_tiles]
-    uint32_t* rbase;            // [capacity_tiles]
+    uint32_t* rbase;            // [capacity_tiles + 1]
     uint32_t* scount;           // [capacity_tiles]
+    uint32_t* sprefix;          // [capacity_tiles + 1] exclusive prefix of scount (scanned passes)
     uint2* survivors;           // [capacity_records]
     uint32_t capacity_tiles;
     uint64_t capacity_records;

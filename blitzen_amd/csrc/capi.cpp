@@ -29,6 +29,7 @@ static_assert(sizeof(bz_Cluster) == 32, "Cluster layout");
 static_assert(sizeof(bz_IndirectDrawData) == 24, "IndirectDrawData layout");
 static_assert(sizeof(bz_ClusterDispatchData) == 12, "ClusterDispatchData layout");
 static_assert(sizeof(bz_CameraViewData) == 256, "CameraViewData layout");
+static_assert(sizeof(bz_result) == 64 && offsetof(bz_result, arrival_ns) == 56, "bz_result layout");
 static_assert(offsetof(bz_CameraViewData, lodTarget) == 180, "CameraViewData.lodTarget offset");
 static_assert(offsetof(bz_CameraViewData, frustumRight) == 140, "CameraViewData.frustumRight offset");
 static_assert(offsetof(bz_Cluster, dataOffset) == 20, "Cluster.dataOffset offset");
@@ -69,11 +70,13 @@ Pyramid* Ctx::pyramid(bz_pyramid h)
 Scene::~Scene()
 {
     void* ps[] = { obj_surf, obj_surf16, obj_xf, xf_ps, xf_q, surf_sphere, surf_lod, lod_error, lod_draw, lod_cluster, vis,
-                   cl_sphere, cl_cone, cl_draw, cache.sph, cache.meta, cache.counts };
+                   cl_sphere, cl_cone, cl_draw, cache.sph, cache.meta, cache.counts, xf_winner };
+    for (XfSlot& x : xf_slot)
+        if (x.done) (void)hipEventSynchronize(x.done);
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (XfSlot& x : xf_slot) {
-        if (x.done) (void)hipEventSynchronize(x.done);
+        if (x.dev_ids) (void)hipFree(x.dev_ids);
         if (x.host) (void)hipHostFree(x.host);
         if (x.done) (void)hipEventDestroy(x.done);
     }
@@ -87,8 +90,8 @@ Pyramid::~Pyramid()
 
 StreamScratch::~StreamScratch()
 {
-    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err, xs.ccounts, xs.rbase, xs.scount, xs.survivors,
-                   inst_hist };
+    void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err, xs.ccounts, xs.rbase, xs.scount, xs.sprefix,
+                   xs.survivors, inst_hist };
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (jobs_host) (void)hipHostFree(jobs_host);
@@ -123,11 +126,12 @@ bz_status Ctx::scratch_for(hipStream_t st, uint32_t tiles, uint64_t entries, Str
         if (s.cs.counts) (void)hipFree(s.cs.counts);
         if (s.cs.staging) (void)hipFree(s.cs.staging);
         s.cs = CompactScratch{};
-        e = hipMalloc((void**)&s.cs.counts, (size_t)cap_t * 4);
+        e = hipMalloc((void**)&s.cs.counts, (size_t)counts_words(cap_t) * 4);
         if (e == hipSuccess) e = hipMalloc((void**)&s.cs.staging, (size_t)cap_e * sizeof(uint2));
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(compaction scratch)");
         s.cs.capacity_tiles = cap_t;
         s.cs.capacity_entries = cap_e;
+        set_prefix(s.cs);
     }
     *out = &s;
     return BZ_OK;
@@ -145,12 +149,13 @@ bz_status Ctx::cluster_scratch_for(hipStream_t st, StreamScratch* sc, uint32_t t
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     const uint32_t cap_t = std::max<uint32_t>(std::max(tiles, x.capacity_tiles), 1u << 10);
     const uint64_t cap_r = std::max<uint64_t>(std::max(records, x.capacity_records), 1ull << 16);
-    for (void* p : { (void*)x.ccounts, (void*)x.rbase, (void*)x.scount, (void*)x.survivors })
+    for (void* p : { (void*)x.ccounts, (void*)x.rbase, (void*)x.scount, (void*)x.sprefix, (void*)x.survivors })
         if (p) (void)hipFree(p);
     x = ClusterScratch{};
     e = hipMalloc((void**)&x.ccounts, (size_t)cap_t * 4);
-    if (e == hipSuccess) e = hipMalloc((void**)&x.rbase, (size_t)cap_t * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&x.rbase, ((size_t)cap_t + 1) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&x.scount, (size_t)cap_t * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&x.sprefix, ((size_t)cap_t + 1) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&x.survivors, (size_t)cap_r * sizeof(uint2));
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(cluster scratch)");
     x.capacity_tiles = cap_t;
@@ -549,11 +554,22 @@ bz_status bz_scene_transform_staging(bz_ctx* ctx, bz_scene scene, uint32_t capac
     }
     if (x.cap < capacity) {
         if (x.host) (void)hipHostFree(x.host);
+        if (x.dev_ids) (void)hipFree(x.dev_ids);
         x.host = nullptr;
+        x.dev_ids = nullptr;
         x.cap = 0;
         const size_t off = ((size_t)capacity * 4 + 255) / 256 * 256;
         const size_t bytes = off + (size_t)capacity * sizeof(bz_MeshTransform);
         hipError_t e = hipHostMalloc((void**)&x.host, bytes, hipHostMallocDefault);  // pinned, device-readable
+        if (e == hipSuccess) e = hipMalloc((void**)&x.dev_ids, (size_t)capacity * 4);
+        if (e == hipSuccess && !s->xf_winner) {
+            e = hipMalloc((void**)&s->xf_winner, (size_t)s->n_transforms * 4);
+            if (e == hipSuccess) e = hipMemset(s->xf_winner, 0, (size_t)s->n_transforms * 4);
+            if (e != hipSuccess && s->xf_winner) {
+                (void)hipFree(s->xf_winner);
+                s->xf_winner = nullptr;
+            }
+        }
         if (e == hipSuccess && !x.done) e = hipEventCreateWithFlags(&x.done, hipEventDisableTiming);
         if (e != hipSuccess) return ctx->hip_fail(e, "transform staging slot");
         x.cap = capacity;
@@ -580,10 +596,11 @@ bz_status bz_scene_commit_transforms(bz_ctx* ctx, bz_scene scene, uint32_t n, vo
     ++s->xf_version;  // frame-cache survivors recorded before this update no longer apply
     hipError_t e = hipSuccess;
     if (n) {
-        // the scatter kernel reads the pinned slot directly over the host link (one pass, no staging copy)
+        // the kernels read the pinned slot directly over the host link (ids once, records once; no staging copy)
         e = launch_scatter_staged_transforms(reinterpret_cast<const uint32_t*>(x.host),
                                              reinterpret_cast<const float4*>(x.host + x.xf_offset), n,
-                                             s->n_transforms, s->xf_ps, s->xf_q, sc->err, st);
+                                             s->n_transforms, x.dev_ids, s->xf_winner, s->xf_ps, s->xf_q, sc->err,
+                                             st);
     }
     if (e == hipSuccess) e = hipEventRecord(x.done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "staged transform update");
@@ -638,6 +655,34 @@ bz_status bz_scene_get_visibility(bz_ctx* ctx, bz_scene scene, uint32_t* vis, ui
     hipError_t e = hipMemcpyAsync(vis, s->vis, (size_t)n * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return ctx->hip_fail(e, "visibility download");
+    return BZ_OK;
+}
+
+bz_status bz_scene_get_transforms(bz_ctx* ctx, bz_scene scene, uint32_t first, uint32_t n, bz_MeshTransform* out,
+                                  void* stream)
+{
+    if (!ctx) return BZ_E_INVALID;
+    Scene* s = ctx->scene(scene);
+    if (!s) return ctx->fail(BZ_E_INVALID, "unknown scene handle");
+    if ((uint64_t)first + n > s->n_transforms || (n && !out)) return ctx->fail(BZ_E_INVALID, "transform range");
+    if (!n) return BZ_OK;
+    std::vector<float4> ps(n), q(n);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(ps.data(), s->xf_ps + first, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(q.data(), s->xf_q + first, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "transform download");
+    for (uint32_t k = 0; k < n; ++k) {
+        out[k].pos[0] = ps[k].x;
+        out[k].pos[1] = ps[k].y;
+        out[k].pos[2] = ps[k].z;
+        out[k].scale = ps[k].w;
+        out[k].orientation[0] = q[k].x;
+        out[k].orientation[1] = q[k].y;
+        out[k].orientation[2] = q[k].z;
+        out[k].orientation[3] = q[k].w;
+    }
     return BZ_OK;
 }
 
@@ -1141,8 +1186,8 @@ bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, v
     if (s) return s;
     s = ctx->upload_jobs(sc, cj.data(), (uint32_t)cj.size(), st);
     if (s) return s;
-    e = launch_cull_batch(sc->jobs_dev, table_starts(sc->jobs_dev, sc->jobs_cap), (uint32_t)cj.size(), tiles, sc->cs,
-                          t.block, t.items, st);
+    e = launch_cull_batch(sc->jobs_dev, table_starts(sc->jobs_dev, sc->jobs_cap), (uint32_t)cj.size(), tiles,
+                          max_job_tiles(cj.data(), cj.size()), sc->cs, t.block, t.items, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "batched cull launch");
     return BZ_OK;
@@ -1219,13 +1264,14 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     // graph-owned device state
     hipError_t e = hipSuccess;
     const uint32_t cap = std::max<uint32_t>(tiles, 1u);
-    e = hipMalloc((void**)&g->cs.counts, (size_t)cap * 4);
+    e = hipMalloc((void**)&g->cs.counts, (size_t)counts_words(cap) * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&g->cs.staging, (size_t)cap * t.block * t.items * sizeof(uint2));
     if (e == hipSuccess) e = hipMalloc((void**)&g->err, 16);
     if (e == hipSuccess) e = hipMemset(g->err, 0, 16);
     if (e != hipSuccess) return ctx->hip_fail(e, "graph scratch");
     g->cs.capacity_tiles = cap;
     g->cs.capacity_entries = (uint64_t)cap * t.block * t.items;
+    set_prefix(g->cs);
     // job tables (descriptors, then the compact start indices), owned by the graph
     if (!cj.empty()) {
         std::vector<uint32_t> starts(cj.size());
@@ -1266,7 +1312,7 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (e == hipSuccess) e = zero_counts(empty, cs);
     if (e == hipSuccess && !cj.empty())
         e = launch_cull_batch(g->jobs_dev, table_starts(g->jobs_dev, (uint32_t)cj.size()), (uint32_t)cj.size(), tiles,
-                              g->cs, t.block, t.items, cs);
+                              max_job_tiles(cj.data(), cj.size()), g->cs, t.block, t.items, cs);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     (void)hipStreamDestroy(cs);

@@ -6,7 +6,8 @@
 // the device and the order is deterministic:
 //   object pass (cull.hip, decide_cluster_kernel): per object tile, the emitting objects (objectId, LOD) in object
 //       order and the tile's cluster total;
-//   expand:  one workgroup per object tile. Its first record is the sum of the earlier tiles' totals; it writes the
+//   expand:  one workgroup per object tile. Its first record is the sum of the earlier tiles' totals (added by the
+//       workgroup itself for up to kScanTiles tiles, else read from a scan of the totals launched before); it writes the
 //       tile's records in (objectId, clusterId) order, tests every record's cluster (frustum, backface cone, Hi-Z)
 //       and compacts the survivors, in record order, into the tile's own record range of a survivor buffer;
 //   emit:    one wave per tile adds the earlier tiles' survivor counts and writes the DrawIndexedIndirect commands
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(kXBlock) void cluster_expand_kernel(ClusterJob J, c
         for (uint32_t L = 0; L < tid; ++L) off += level_dim(J.pyr.base_w, L) * level_dim(J.pyr.base_h, L);
         sm.lvl[tid] = off;
     }
-    const uint32_t base = block_prefix(ccounts, t, sm.red);
+    const uint32_t base = J.scanned ? rbase[t] : block_prefix(ccounts, t, sm.red);
 
     // the tile's emitting objects -> LOD cluster ranges, inclusive scan of the cluster counts
     uint32_t carry = 0;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kXBlock) void cluster_expand_kernel(ClusterJob J, c
     }
     const uint32_t T = carry;  // == ccounts[t]
     if (tid == 0) {
-        rbase[t] = base;
+        if (!J.scanned) rbase[t] = base;
         if (t == J.n_tiles - 1u) *J.dispatch_count = base + T;
     }
 
@@ -205,7 +206,8 @@ constexpr uint32_t kCEmitTiles = kXBlock / 64;
 constexpr uint32_t kCEmitUnroll = 4;
 
 __global__ __launch_bounds__(kXBlock) void cluster_emit_kernel(ClusterJob J, const uint32_t* rbase,
-                                                               const uint32_t* scount, const uint2* survivors)
+                                                               const uint32_t* scount, const uint32_t* sprefix,
+                                                               const uint2* survivors)
 {
     __shared__ uint32_t red[kXWaves];
     __shared__ uint32_t cnt[kCEmitTiles + 1];
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(kXBlock) void cluster_emit_kernel(ClusterJob J, con
     const uint32_t t1 = t0 + kCEmitTiles < J.n_tiles ? t0 + kCEmitTiles : J.n_tiles;
     const uint32_t nt = t1 - t0;
     const uint32_t own = tid < nt ? scount[t0 + tid] : 0u;
-    const uint32_t prefix = block_prefix(scount, t0, red);
+    const uint32_t prefix = J.scanned ? sprefix[t0] : block_prefix(scount, t0, red);
     if (wave == 0) {
         uint32_t v = own;
 #pragma unroll
@@ -256,17 +258,24 @@ __global__ __launch_bounds__(kXBlock) void cluster_emit_kernel(ClusterJob J, con
 
 }  // namespace
 
-hipError_t launch_cluster_pass(const ClusterJob& job, const CompactScratch& cs, const ClusterScratch& xs,
+hipError_t launch_cluster_pass(const ClusterJob& job_in, const CompactScratch& cs, const ClusterScratch& xs,
                                hipStream_t st)
 {
-    if (job.n_tiles == 0 || job.tile_size > kXMaxTile) return hipErrorInvalidValue;
+    if (job_in.n_tiles == 0 || job_in.tile_size > kXMaxTile) return hipErrorInvalidValue;
+    ClusterJob job = job_in;
+    job.scanned = job.n_tiles > kScanTiles ? 1u : 0u;
+    hipError_t e = hipSuccess;
+    if (job.scanned) e = launch_scan_counts(xs.ccounts, job.n_tiles, xs.rbase, st);  // record bases
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(cluster_expand_kernel, dim3(job.n_tiles), dim3(kXBlock), 0, st, job,
                        (const uint32_t*)cs.counts, (const uint2*)cs.staging, (const uint32_t*)xs.ccounts, xs.rbase,
                        xs.scount, xs.survivors);
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess && job.scanned) e = launch_scan_counts(xs.scount, job.n_tiles, xs.sprefix, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(cluster_emit_kernel, dim3((job.n_tiles + kCEmitTiles - 1) / kCEmitTiles), dim3(kXBlock), 0, st,
-                       job, (const uint32_t*)xs.rbase, (const uint32_t*)xs.scount, (const uint2*)xs.survivors);
+                       job, (const uint32_t*)xs.rbase, (const uint32_t*)xs.scount, (const uint32_t*)xs.sprefix,
+                       (const uint2*)xs.survivors);
     return hipGetLastError();
 }
 

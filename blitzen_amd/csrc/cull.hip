@@ -597,12 +597,14 @@ __device__ __forceinline__ void block_lds_sync()
 }
 
 // Writes the commands of tiles [t0, t1) of one job (t1 - t0 <= kEmitTiles; tile_size >= 64 * kEmitUnroll). One round
-// of loads gets the counts of the job's earlier tiles (reduced to the output base), the workgroup's own counts and,
+// of loads gets the counts of the job's earlier tiles (reduced to the output base; at most kScanTiles of them), or
+// with `scanned` (the exclusive prefix of all counts) two entries of it, the workgroup's own counts and,
 // before any count is known, the first 64 * kEmitUnroll staging entries of each wave's tile (in bounds: every tile
 // owns tile_size slots); each wave gathers its entries' LOD draw data while the block scan of the counts runs, then
 // writes tile t0 + w's commands in order. Longer tiles continue in rounds of kEmitUnroll entries per lane.
 __device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32_t t1, uint32_t tile_size,
-                                           const uint32_t* counts, const uint2* staging, EmitSmem& sm)
+                                           const uint32_t* counts, const uint2* staging, const uint32_t* scanned,
+                                           EmitSmem& sm)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t nt = t1 - t0;
@@ -616,7 +618,9 @@ __device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32
     // the job's earlier tiles: every load of a thread issued before the first add (16 per pass)
     uint32_t part = 0;
     const uint32_t own = tid < nt ? counts[t0 + tid] : 0u;
-    for (uint32_t u0 = J.tile_start; u0 < t0; u0 += 16u * kEmitBlock) {
+    if (scanned) {  // one lane takes the difference of the scanned prefixes; the reduction below adds it
+        if (tid == 0) part = scanned[t0] - scanned[J.tile_start];
+    } else for (uint32_t u0 = J.tile_start; u0 < t0; u0 += 16u * kEmitBlock) {
         uint32_t v[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
@@ -674,19 +678,20 @@ __device__ __forceinline__ void emit_tiles(const CullJob& J, uint32_t t0, uint32
 }
 
 __global__ __launch_bounds__(kEmitBlock) void emit_kernel(CullJob J, uint32_t tile_size, const uint32_t* counts,
-                                                          const uint2* staging)
+                                                          const uint2* staging, const uint32_t* prefix)
 {
     __shared__ EmitSmem sm;
     const uint32_t t0 = blockIdx.x * kEmitTiles;
     const uint32_t t1 = t0 + kEmitTiles < J.n_tiles ? t0 + kEmitTiles : J.n_tiles;
-    emit_tiles(J, t0, t1, tile_size, counts, staging, sm);
+    emit_tiles(J, t0, t1, tile_size, counts, staging, prefix, sm);
 }
 
 // Batched emit: workgroup b covers global tiles [b*kEmitTiles, ...), split at job boundaries.
 __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* __restrict__ jobs,
                                                                 const uint32_t* __restrict__ starts, uint32_t n_jobs,
                                                                 uint32_t total_tiles, uint32_t tile_size,
-                                                                const uint32_t* counts, const uint2* staging)
+                                                                const uint32_t* counts, const uint2* staging,
+                                                                const uint32_t* prefix)
 {
     __shared__ EmitSmem sm;
     uint32_t t = blockIdx.x * kEmitTiles;
@@ -698,9 +703,24 @@ __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* _
         const CullJob& J = jobs[sm.job];
         const uint32_t job_end = J.tile_start + J.n_tiles;
         const uint32_t t1 = job_end < t_end ? job_end : t_end;
-        emit_tiles(J, t, t1, tile_size, counts, staging, sm);
+        emit_tiles(J, t, t1, tile_size, counts, staging, prefix, sm);
         t = t1;
     }
+}
+
+// The emit launch of one job whose decide launch wrote s.counts / s.staging; a job of more than kScanTiles tiles
+// first gets its counts' prefix from the scan.
+hipError_t launch_emit(const CullJob& job, uint32_t tile_size, const CompactScratch& s, hipStream_t st)
+{
+    const uint32_t* prefix = nullptr;
+    if (job.n_tiles > kScanTiles) {
+        const hipError_t e = launch_scan_counts(s.counts, job.n_tiles, s.prefix, st);
+        if (e != hipSuccess) return e;
+        prefix = s.prefix;
+    }
+    hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
+                       tile_size, (const uint32_t*)s.counts, (const uint2*)s.staging, prefix);
+    return hipGetLastError();
 }
 
 // LATE pass from a frame cache: the EARLY pass of the same view, transforms and range recorded each tile's frustum
@@ -711,10 +731,12 @@ __global__ __launch_bounds__(kEmitBlock) void emit_batch_kernel(const CullJob* _
 // LATE pass, so the commands, count and visibility are identical.
 constexpr uint32_t kLcBlock = 512;  // threads per workgroup of the cached late pass (kSuperTile objects each)
 
-// A workgroup takes a super-tile of kSuperTile objects (kSuperTile / TILE recorded tiles): the visibility of its
-// objects as a bitmap in LDS, the survivors of its tiles with their LOD, and the Hi-Z test; then the visibility of
-// the whole super-tile (coalesced; 0 outside the frustum) and the newly visible survivors' commands in survivor
-// order. No transform, surface or LOD table is read.
+// A workgroup takes a super-tile of kSuperTile objects (kSuperTile / TILE recorded tiles): the survivors of its
+// tiles with their LOD, each survivor's current visibility (gathered in flight with its Hi-Z texels: only
+// survivors can be emitted) and the Hi-Z test; then the visibility of the whole super-tile (coalesced; 0 outside
+// the frustum) and the newly visible survivors' commands in survivor order. No transform, surface or LOD table is
+// read. (A coalesced load of the whole super-tile's visibility instead took 8 more registers per thread, which cost
+// a quarter of the resident workgroups: 25.9 us against 14.6 us per C1 pass.)
 template <int TILE>
 __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameCacheDev fc, uint32_t n_rec_tiles,
                                                                uint32_t* counts, uint2* staging)
@@ -723,7 +745,7 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
     constexpr uint32_t S = kSuperTile / TILE;            // recorded tiles per super-tile
     constexpr uint32_t OPT = kSuperTile / BLOCK;         // objects per thread
     constexpr uint32_t U = 2;                            // survivors per thread in the first round
-    __shared__ uint32_t s_new[kSuperTile / 32], s_old[kSuperTile / 32];
+    __shared__ uint32_t s_new[kSuperTile / 32];
     __shared__ uint32_t s_lvl[kMaxMips], s_cnt[S + 1], s_woff[U][NW], s_base;
     const uint32_t st = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t base = st * kSuperTile;  // first object of the super-tile, relative to the job's range
@@ -746,13 +768,6 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
         const uint32_t t = st * S + tid;
         cnt = t < n_rec_tiles ? fc.counts[t] : 0u;
     }
-    // the super-tile's current visibility (one coalesced load per thread and item, in flight with the records)
-    uint32_t old[OPT];
-#pragma unroll
-    for (uint32_t k = 0; k < OPT; ++k) {
-        const uint32_t i = k * BLOCK + tid;
-        old[k] = i < nt ? J.vis_in[obj0 + i] : 0u;
-    }
     if (wave == 0) {  // prefix of the recorded tiles' counts (S <= 8 lanes)
         uint32_t x = lane < S ? cnt : 0u;
 #pragma unroll
@@ -770,15 +785,6 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
     }
     for (uint32_t x = tid; x < kSuperTile / 32; x += BLOCK) s_new[x] = 0u;
     if (tid == 0) s_base = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < OPT; ++k) {  // objects k * BLOCK + wave * 64 + lane: two bitmap words per wave and item
-        const unsigned long long m = __ballot(old[k] != 0u);
-        if (lane == 0) {
-            const uint32_t w = (k * BLOCK + wave * 64u) >> 5;
-            s_old[w] = (uint32_t)m;
-            s_old[w + 1] = (uint32_t)(m >> 32);
-        }
-    }
     __syncthreads();
     const uint32_t total = s_cnt[S];
     uint2* stage = staging + (size_t)st * kSuperTile;
@@ -793,6 +799,7 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
             lod_abs[k] = 0u;
             const uint32_t local = mt[k].x - obj0;
             if (have[k] && local < nt) {  // (an entry outside the super-tile can only come from a mismatched cache)
+                const uint32_t last = J.vis_in[mt[k].x];  // the current visibility, read now (LateDrawCull:51)
                 float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
                 const bool test = project_sphere(sp[k].x, sp[k].y, sp[k].z, sp[k].w, J.view.znear, J.view.p00,
                                                  J.view.p11, a0, a1, a2, a3);
@@ -800,7 +807,7 @@ __global__ __launch_bounds__(kLcBlock) void late_cached_kernel(CullJob J, FrameC
                 const bool vis =
                     test ? occlusion_visible(J.view, J.pyr, s_lvl, a0, a1, a2, a3, sp[k].z, sp[k].w) : true;
                 if (vis) atomicOr(&s_new[local >> 5], 1u << (local & 31u));
-                emit[k] = vis && ((s_old[local >> 5] >> (local & 31u)) & 1u) == 0u;
+                emit[k] = vis && last == 0u;
                 lod_abs[k] = mt[k].y;
             }
         }
@@ -874,9 +881,7 @@ hipError_t launch_late_cached_t(const CullJob& job, const CompactScratch& s, con
                        s.counts, s.staging);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
-                       kSuperTile, (const uint32_t*)s.counts, (const uint2*)s.staging);
-    return hipGetLastError();
+    return launch_emit(job, kSuperTile, s, st);
 }
 
 // Persistent grid: `per_cu` workgroups per CU, or (per_cu == 0) as many as are resident at once per CU.
@@ -935,22 +940,26 @@ hipError_t launch_cull_t(const CullJob& job, const CompactScratch& s, uint32_t n
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !with_emit) return e;
-    hipLaunchKernelGGL(emit_kernel, dim3((job.n_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st, job,
-                       (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts, (const uint2*)s.staging);
-    return hipGetLastError();
+    return launch_emit(job, (uint32_t)(BLOCK * ITEMS), s, st);
 }
 
 template <int BLOCK, int ITEMS>
 hipError_t launch_cull_batch_t(const CullJob* jobs, const uint32_t* starts, uint32_t n_jobs, uint32_t total_tiles,
-                               const CompactScratch& s, hipStream_t st)
+                               uint32_t max_job_tiles, const CompactScratch& s, hipStream_t st)
 {
     hipLaunchKernelGGL((decide_batch_kernel<BLOCK, ITEMS>), dim3(total_tiles), dim3(BLOCK), 0, st, jobs, starts,
                        n_jobs, s.counts, s.staging);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const uint32_t* prefix = nullptr;
+    if (max_job_tiles > kScanTiles) {  // the batch's global prefix; each job takes differences of it
+        e = launch_scan_counts(s.counts, total_tiles, s.prefix, st);
+        if (e != hipSuccess) return e;
+        prefix = s.prefix;
+    }
     hipLaunchKernelGGL(emit_batch_kernel, dim3((total_tiles + kEmitTiles - 1) / kEmitTiles), dim3(kEmitBlock), 0, st,
                        jobs, starts, n_jobs, total_tiles, (uint32_t)(BLOCK * ITEMS), (const uint32_t*)s.counts,
-                       (const uint2*)s.staging);
+                       (const uint2*)s.staging, prefix);
     return hipGetLastError();
 }
 
@@ -1006,14 +1015,69 @@ hipError_t launch_cluster_objects(const CullJob& job, const CompactScratch& s, u
 }
 
 hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs, uint32_t total_tiles,
-                             const CompactScratch& s, uint32_t block, uint32_t items, hipStream_t st)
+                             uint32_t max_job_tiles, const CompactScratch& s, uint32_t block, uint32_t items,
+                             hipStream_t st)
 {
     if (n_jobs == 0 || total_tiles == 0) return hipSuccess;
-#define X(B, I) \
-    if (block == B && items == I) return launch_cull_batch_t<B, I>(jobs_dev, starts_dev, n_jobs, total_tiles, s, st);
+#define X(B, I)                                                                                                      \
+    if (block == B && items == I)                                                                                     \
+        return launch_cull_batch_t<B, I>(jobs_dev, starts_dev, n_jobs, total_tiles, max_job_tiles, s, st);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------------------------------------
+// exclusive prefix of tile counts (jobs of more than kScanTiles tiles)
+// ------------------------------------------------------------------------------------------------
+constexpr uint32_t kScanBlock = 1024;
+constexpr uint32_t kScanPer = 8;  // consecutive counts per thread and round
+
+// One workgroup walks the counts in rounds of kScanBlock * kScanPer: per thread an exclusive scan of its 8 counts,
+// a wave scan of the thread totals, the waves' totals through LDS, and the carry of the earlier rounds. Linear in n;
+// a launch of its own because it needs every count of the job.
+__global__ __launch_bounds__(kScanBlock) void scan_counts_kernel(const uint32_t* __restrict__ counts, uint32_t n,
+                                                                 uint32_t* __restrict__ prefix)
+{
+    constexpr uint32_t NW = kScanBlock / 64;
+    __shared__ uint32_t wsum[NW];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += kScanBlock * kScanPer) {
+        const uint32_t i0 = r0 + tid * kScanPer;
+        uint32_t v[kScanPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) v[k] = i0 + k < n ? counts[i0 + k] : 0u;
+        uint32_t own = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t c = v[k];
+            v[k] = own;
+            own += c;
+        }
+        const uint32_t incl = wave_incl_scan(own, lane);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; ++w) {
+            before += w < wave ? wsum[w] : 0u;
+            total += wsum[w];
+        }
+        const uint32_t base = carry + before + (incl - own);
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k)
+            if (i0 + k < n) prefix[i0 + k] = base + v[k];
+        carry += total;
+        __syncthreads();  // wsum is rewritten by the next round
+    }
+    if (threadIdx.x == 0) prefix[n] = carry;
+}
+
+hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* prefix, hipStream_t st)
+{
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(kScanBlock), 0, st, counts, n, prefix);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1034,27 +1098,52 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
     return hipGetLastError();
 }
 
-__global__ void scatter_staged_transforms_kernel(const uint32_t* __restrict__ ids, const float4* __restrict__ records,
-                                                 uint32_t n, uint32_t n_xf, float4* xf_ps, float4* xf_q, uint32_t* err)
+// Transform updates, in two launches so that an id named more than once in one commit takes its last occurrence
+// (UpdateObjectTransform copies updates into the staging buffer in call order, vulkanDraw.cpp:816-820):
+//   mark:    update i copies its id from the pinned slot into device memory and raises winner[id] to i + 1;
+//   scatter: update i writes its 32-byte record into the position/scale and quaternion streams only if it is the
+//            winner, which then clears winner[id] (the array is all zero between commits; a non-winner compares its
+//            own i + 1 and never matches, whether it reads before or after that clear).
+// Ids >= n_xf are skipped and set err[2].
+__global__ void mark_staged_transforms_kernel(const uint32_t* __restrict__ ids, uint32_t n, uint32_t n_xf,
+                                              uint32_t* dev_ids, uint32_t* winner, uint32_t* err)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = ids[i];
-    const float4 ps = records[2u * i], q = records[2u * i + 1u];
+    dev_ids[i] = id;
     if (id >= n_xf) {
         atomicOr(&err[2], 1u);
         return;
     }
+    atomicMax(&winner[id], i + 1u);
+}
+
+__global__ void scatter_staged_transforms_kernel(const uint32_t* __restrict__ dev_ids,
+                                                 const float4* __restrict__ records, uint32_t n, uint32_t n_xf,
+                                                 uint32_t* winner, float4* xf_ps, float4* xf_q)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = dev_ids[i];
+    if (id >= n_xf || winner[id] != i + 1u) return;
+    const float4 ps = records[2u * i], q = records[2u * i + 1u];
     xf_ps[id] = ps;
     xf_q[id] = q;
+    winner[id] = 0u;
 }
 
 hipError_t launch_scatter_staged_transforms(const uint32_t* ids, const float4* records, uint32_t n, uint32_t n_xf,
-                                            float4* xf_ps, float4* xf_q, uint32_t* err, hipStream_t st)
+                                            uint32_t* dev_ids, uint32_t* winner, float4* xf_ps, float4* xf_q,
+                                            uint32_t* err, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(scatter_staged_transforms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, ids, records, n, n_xf,
-                       xf_ps, xf_q, err);
+    const dim3 g((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(mark_staged_transforms_kernel, g, b, 0, st, ids, n, n_xf, dev_ids, winner, err);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(scatter_staged_transforms_kernel, g, b, 0, st, (const uint32_t*)dev_ids, records, n, n_xf,
+                       winner, xf_ps, xf_q);
     return hipGetLastError();
 }
 

@@ -163,6 +163,13 @@ class Context:
         self.check(lib().bz_scene_visibility_ptr(self.handle, scene, C.byref(p)), "bz_scene_visibility_ptr")
         return p.value
 
+    def get_transforms(self, scene: int, first: int, n: int, stream=None) -> np.ndarray:
+        """The resident transforms [first, first + n) in the reference's MeshTransform layout."""
+        out = np.empty(n, dtype=MESH_TRANSFORM)
+        self.check(lib().bz_scene_get_transforms(self.handle, scene, first, n, A.ptr(out), stream),
+                   "bz_scene_get_transforms")
+        return out
+
     def frame_cache_stats(self, scene: int) -> tuple[int, int]:
         """(EARLY passes that recorded survivors, LATE passes that read them) for the scene."""
         rec, reu = C.c_uint64(), C.c_uint64()

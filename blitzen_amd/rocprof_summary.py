@@ -8,6 +8,11 @@
         per C1 frame (every dispatch from one pyramid build to the next), from a FETCH_SIZE and a WRITE_SIZE counter
         run (CSV output). Each counter is scaled by what it reads for the runtime's device-to-device copy kernel of
         N MiB in the same run (FETCH_SIZE = N MiB read, WRITE_SIZE = N MiB written).
+    python -m blitzen_amd.rocprof_summary counters <a.csv> [<b.csv> ...] [--calib-mib N] [--title T] [--command C]
+                                              [--out OUT]
+        per kernel, the median per dispatch of every counter in the CSV counter-collection files (one counter set per
+        run); FETCH_SIZE / WRITE_SIZE also in bytes, scaled by the calibration copy when --calib-mib is given.
+Every summary records the digest of the kernel sources it measured (blitzen_amd.build.kernel_sources_digest).
 """
 from __future__ import annotations
 
@@ -51,6 +56,12 @@ def demangle_short(name: str) -> str:
     return n
 
 
+def sources_digest() -> str:
+    from blitzen_amd.build import kernel_sources_digest
+
+    return kernel_sources_digest()
+
+
 def trace(db: str, title: str, command: str) -> str:
     c = sqlite3.connect(db)
     rows = c.execute("select s.kernel_name, d.start, d.end from rocpd_kernel_dispatch d "
@@ -58,7 +69,8 @@ def trace(db: str, title: str, command: str) -> str:
     agg = defaultdict(list)
     for name, st, en in rows:
         agg[name].append((en - st) / 1000.0)
-    lines = [f"# {title}", "", "Command:", "", "```", command, "```", "",
+    lines = [f"# {title}", "", f"Kernel sources: `{sources_digest()}` (blitzen_amd.build.kernel_sources_digest)", "",
+             "Command:", "", "```", command, "```", "",
              "| kernel | dispatches | avg us | min us | max us | total us | symbol |", "|---|---|---|---|---|---|---|"]
     for name, ds in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
         lines.append(f"| {demangle_short(name)} | {len(ds)} | {statistics.mean(ds):.2f} | {min(ds):.2f} | "
@@ -135,6 +147,7 @@ def frame_traffic(fetch_csv: str, write_csv: str, calib_mib: float, command: str
         "calibration": f"{COPY_KERNEL} of {calib_mib} MiB: FETCH_SIZE median {statistics.median(f_copy):.1f} KiB, "
                        f"WRITE_SIZE median {statistics.median(w_copy):.1f} KiB",
         "frames": min(len(ff), len(ww)),
+        "kernel_sources": sources_digest(),
         "command": command,
     }
 
@@ -165,8 +178,55 @@ def traffic(fetch_csv: str, write_csv: str, calib_mib: float, command: str):
         "calibration": f"{COPY_KERNEL} of {calib_mib} MiB: FETCH_SIZE median {statistics.median(f_copy):.1f} KiB, "
                        f"WRITE_SIZE median {statistics.median(w_copy):.1f} KiB",
         "late_passes": min(len(lf), len(lw)),
+        "kernel_sources": sources_digest(),
         "command": command,
     }
+
+
+def read_all_counters(path: str):
+    """{(dispatch_id, counter): (kernel_name, value)} for every counter of a CSV counter-collection file."""
+    out = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            key = (int(row["Dispatch_Id"]), row["Counter_Name"])
+            prev = out.get(key)
+            out[key] = (row["Kernel_Name"], (prev[1] if prev else 0.0) + float(row["Counter_Value"]))
+    return out
+
+
+def counters(paths, calib_mib: float, title: str, command: str) -> str:
+    per = defaultdict(lambda: defaultdict(list))  # kernel -> counter -> values per dispatch
+    for p in paths:
+        for (_, cname), (kname, val) in read_all_counters(p).items():
+            per[kname][cname].append(val)
+    scale = {}
+    if calib_mib:
+        known_kib = calib_mib * 1024.0
+        for cname in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals = [v for k, cs in per.items() if COPY_KERNEL in k for v in cs.get(cname, [])
+                    if v > 0.5 * known_kib * 0.25]
+            if vals:
+                scale[cname] = known_kib / statistics.median(vals)
+    names = sorted({c for cs in per.values() for c in cs})
+    lines = [f"# {title}", "", f"Kernel sources: `{sources_digest()}` (blitzen_amd.build.kernel_sources_digest)", "",
+             "Command:", "", "```", command, "```", "",
+             "Median per dispatch." + (" FETCH_SIZE / WRITE_SIZE in KiB as read, and scaled to bytes by the "
+                                       "calibration copy: " + ", ".join(f"{k} x{v:.4f}" for k, v in scale.items())
+                                       if scale else ""), "",
+             "| kernel | dispatches | " + " | ".join(names) + (" | fetch bytes | write bytes |" if scale else " |"),
+             "|---|---|" + "---|" * len(names) + ("---|---|" if scale else "")]
+    for kname, cs in sorted(per.items(), key=lambda kv: kv[0]):
+        if "rocclr" in kname or "fill_u32" in kname:
+            continue
+        n = max(len(v) for v in cs.values())
+        row = [demangle_short(kname), str(n)]
+        for c in names:
+            row.append(f"{statistics.median(cs[c]):.1f}" if cs.get(c) else "")
+        if scale:
+            for c in ("FETCH_SIZE", "WRITE_SIZE"):
+                row.append(f"{statistics.median(cs[c]) * 1024 * scale[c]:.0f}" if cs.get(c) and c in scale else "")
+        lines.append("| " + " | ".join(row) + " |")
+    return "\n".join(lines) + "\n"
 
 
 def main(argv=None):
@@ -184,7 +244,19 @@ def main(argv=None):
     r.add_argument("--command", default="")
     r.add_argument("--json")
     r.add_argument("--frame", action="store_true", help="per C1 frame instead of per late pass")
+    k = sub.add_parser("counters")
+    k.add_argument("csv", nargs="+")
+    k.add_argument("--calib-mib", type=float, default=0.0)
+    k.add_argument("--title", default="Counters per kernel")
+    k.add_argument("--command", default="")
+    k.add_argument("--out")
     a = ap.parse_args(argv)
+    if a.cmd == "counters":
+        text = counters(a.csv, a.calib_mib, a.title, a.command)
+        if a.out:
+            Path(a.out).write_text(text)
+        sys.stdout.write(text)
+        return
     if a.cmd == "trace":
         text = trace(a.db, a.title, a.command)
         if a.out:

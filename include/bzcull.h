@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define BZCULL_ABI_VERSION 1u
+#define BZCULL_ABI_VERSION 2u
 
 #if defined(__GNUC__)
 #define BZ_API __attribute__((visibility("default")))
@@ -173,7 +173,9 @@ BZ_API bz_status bz_scene_create(bz_ctx* ctx,
 BZ_API bz_status bz_scene_set_clusters(bz_ctx* ctx, bz_scene scene, const bz_Cluster* clusters, uint32_t n_clusters);
 BZ_API bz_status bz_scene_destroy(bz_ctx* ctx, bz_scene scene);
 /* Replaces UpdateObjectTransform + UpdateBuffers (vulkanDraw.cpp:816-820, 46-60): scatters n transforms
- * (host arrays) into the resident transform streams, ordered on `stream`. Ids must be < n_transforms and distinct.
+ * (host arrays) into the resident transform streams, ordered on `stream`. Ids must be < n_transforms (checked: the
+ * call fails with nothing applied); an id named more than once takes its last occurrence, as the reference's
+ * sequential copies into its staging buffer do.
  * As in the reference (updates between frames), a cull reading the scene on another stream, including the serving
  * loop's, is not ordered with the update: order them with the streams, or update between frames. */
 BZ_API bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const uint32_t* transform_ids,
@@ -183,7 +185,8 @@ BZ_API bz_status bz_scene_update_transforms(bz_ctx* ctx, bz_scene scene, const u
  * host arrays of `capacity` transform ids and MeshTransforms (32 B records, the reference layout). The caller fills the
  * first n entries and calls bz_scene_commit_transforms: a kernel on `stream` reads them straight from the pinned slot
  * and scatters them (no host copy or conversion). Two slots are used in turn: the caller fills one while the other's
- * scatter runs; handing a slot out again waits for its previous scatter. Ids within one commit must be distinct.
+ * scatter runs; handing a slot out again waits for its previous scatter. An id named more than once in one commit
+ * takes its last occurrence (entries are applied as if in order).
  * Calling the staging function again before committing returns the same slot (reallocated, without its contents,
  * when the new capacity is larger). Ids are checked on the device: an id >= n_transforms is skipped and
  * bz_check(stream) reports BZ_E_INVALID. */
@@ -194,6 +197,10 @@ BZ_API bz_status bz_scene_commit_transforms(bz_ctx* ctx, bz_scene scene, uint32_
 BZ_API bz_status bz_scene_set_visibility(bz_ctx* ctx, bz_scene scene, const uint32_t* visibility, uint32_t n, void* stream);
 /* Copies the visibility buffer to a host array (synchronizes `stream`). */
 BZ_API bz_status bz_scene_get_visibility(bz_ctx* ctx, bz_scene scene, uint32_t* visibility, uint32_t n, void* stream);
+/* Copies the resident transforms [first, first + n) back to a host array in the reference layout (synchronizes
+ * `stream`). */
+BZ_API bz_status bz_scene_get_transforms(bz_ctx* ctx, bz_scene scene, uint32_t first, uint32_t n,
+                                         bz_MeshTransform* transforms, void* stream);
 /* Device pointer of the scene's visibility buffer (uint32[n_objects]). */
 BZ_API bz_status bz_scene_visibility_ptr(bz_ctx* ctx, bz_scene scene, uint32_t** visibility_dev);
 BZ_API bz_status bz_scene_info(bz_ctx* ctx, bz_scene scene, uint32_t* n_objects, uint32_t* n_transforms,
@@ -415,13 +422,15 @@ typedef struct bz_request {
 typedef struct bz_result {
     bz_status status;
     uint32_t count;                /* number of commands */
-    uint64_t submit_ns;            /* CLOCK_MONOTONIC timestamps */
-    uint64_t admit_ns;
-    uint64_t complete_ns;
+    uint64_t submit_ns;            /* CLOCK_MONOTONIC timestamps: entered the queue ... */
+    uint64_t admit_ns;             /* ... admitted into a batch (budget reserved) ... */
+    uint64_t complete_ns;          /* ... batch finished on the device */
     uint64_t batch_id;             /* which device batch served the request */
     uint32_t batch_size;           /* requests in that batch */
     uint32_t reserved;
     uint64_t user_tag;
+    uint64_t arrival_ns;           /* when the request was due: its scheduled time for bz_server_submit_schedule
+                                      (submit_ns - arrival_ns is the submitter's own delay), else submit_ns */
 } bz_result;
 
 BZ_API bz_status bz_server_start(bz_ctx* ctx, const bz_server_cfg* cfg);

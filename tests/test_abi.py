@@ -35,6 +35,6 @@ def test_binding_covers_every_declared_symbol():
 
 def test_no_device_calls_without_gpu():
     lib = _capi.load()
-    assert lib.bz_abi_version() == 1
+    assert lib.bz_abi_version() == 2
     assert lib.bz_status_str(0) == b"ok"
     assert lib.bz_status_str(-4) == b"output capacity exceeded"

@@ -84,7 +84,7 @@ def test_single_request_matches_oracle(ctx, served):
         exp, ne = oracle_for(req, sc)
         assert n == ne
         np.testing.assert_array_equal(got, exp)
-        assert r.submit_ns <= r.admit_ns <= r.complete_ns
+        assert r.arrival_ns == r.submit_ns <= r.admit_ns <= r.complete_ns
 
 
 def test_same_alone_and_in_the_mix(ctx, served):
@@ -115,6 +115,10 @@ def test_scheduled_arrivals(ctx, served):
         np.testing.assert_array_equal(res[k][0], exp)
     sub = [r.submit_ns for _, _, r in res]
     assert all(b >= a for a, b in zip(sub, sub[1:]))
+    # arrivals are the scheduled times: offsets apart, never after the actual submission
+    arr = np.array([r.arrival_ns for _, _, r in res], np.uint64)
+    np.testing.assert_array_equal(arr - arr[0], offsets - offsets[0])
+    assert all(r.arrival_ns <= r.submit_ns for _, _, r in res)
 
 
 def test_deadline_cancel_release_stats(ctx, served):

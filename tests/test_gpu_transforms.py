@@ -130,3 +130,41 @@ def test_staged_update_invalidates_frame_cache(ctx):
         b.free()
     ctx.pyramid_destroy(p)
     ctx.scene_destroy(h)
+
+
+@pytest.mark.parametrize("staged", [False, True])
+def test_duplicate_ids_last_occurrence_wins(ctx, staged):
+    """Ids named several times in one update, in no particular order: every transform ends up as its last occurrence
+    (the reference copies updates in call order, vulkanDraw.cpp:816-820), never a mix of two records, on both the
+    host-array path and the staged path; the cull after it equals the oracle's on those transforms."""
+    n = 40000
+    sc = H.c1_scene(n)
+    xf = sc.transforms.copy()
+    h = ctx.scene_create(sc.objects, xf, sc.surfaces, sc.lods)
+    rng = np.random.default_rng(41 + staged)
+    for frame in range(3):
+        ids = rng.integers(0, 300, 5000).astype(np.uint32)  # ~17 updates per id among 300 ids
+        ids[-50:] = rng.integers(0, n, 50)                  # and some unique ones
+        new = np.zeros(len(ids), dtype=xf.dtype)
+        new["pos"] = rng.uniform(-60, 60, (len(ids), 3)).astype(np.float32)
+        new["pos"][:, 2] = rng.uniform(1, 120, len(ids))
+        new["scale"] = rng.uniform(0.5, 2, len(ids))
+        q = rng.normal(size=(len(ids), 4)).astype(np.float32)
+        new["orientation"] = q / np.linalg.norm(q, axis=1, keepdims=True)
+        if staged:
+            sid, sxf = ctx.transform_staging(h, len(ids))
+            sid[:] = ids
+            sxf[:] = new
+            ctx.commit_transforms(h, len(ids))
+        else:
+            ctx.update_transforms(h, ids, new)
+        ctx.check_kernels()
+        for i, t in zip(ids, new):  # in order: the last occurrence stays
+            xf[i] = t
+        np.testing.assert_array_equal(ctx.get_transforms(h, 0, n).view(np.uint32), xf.view(np.uint32))
+    v = H.view()
+    got, ng = plain(ctx, h, v, n)
+    exp, ne = orc.cull((sc.objects, xf, sc.surfaces, sc.lods), H.view_bytes(v), A.PASS_PLAIN)
+    assert ng == ne
+    np.testing.assert_array_equal(got, H.cmds_u32(exp))
+    ctx.scene_destroy(h)
