@@ -156,6 +156,7 @@ SIGNATURES = {
     "bz_pyramid_reference_extent": (None, [_U32, _U32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
     "bz_pyramid_build_batch": (_I32, [_VP, C.POINTER(PyramidJob), _U32, _VP]),
     "bz_cull": (_I32, [_VP, C.POINTER(CullArgs), _VP]),
+    "bz_cull_noalloc": (_I32, [_VP, C.POINTER(CullArgs), _VP]),
     "bz_cull_instanced": (_I32, [_VP, C.POINTER(InstancedCullArgs), _VP]),
     "bz_scene_transform_staging": (_I32, [_VP, _U64, _U32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "bz_scene_commit_transforms": (_I32, [_VP, _U64, _U32, _VP]),

@@ -19,7 +19,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libbzcull.so"
 
 SOURCES = ["cull.hip", "pyramid.hip", "cluster.hip", "instanced.hip", "capi.cpp", "camera.cpp", "synth.cpp",
-           "autotune.cpp", "server.cpp", "interop.cpp"]
+           "autotune.cpp", "server.cpp", "interop.cpp", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

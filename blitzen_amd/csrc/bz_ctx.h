@@ -149,6 +149,7 @@ struct Ctx {
     std::unordered_map<uintptr_t, std::unique_ptr<StreamScratch>> scratch;
     bz_tuning tune[2] = {};
     std::unique_ptr<Server> server;
+    void* noalloc_scratch = nullptr;  // device address of this device's fixed scratch (bz_cull_noalloc)
 
     Ctx();
     ~Ctx();
@@ -175,6 +176,8 @@ struct Ctx {
 };
 
 ViewConsts make_view_consts(const bz_CameraViewData& v);
+// Device address of the fixed scratch array on `device` (the current device), looked up once per device.
+hipError_t noalloc_scratch_address(int device, void** out);
 
 // Job tables live in one allocation: cap descriptors, then cap uint32 first-tile (or first-workgroup) indices, which
 // the batched kernels stage in LDS for their job lookup. The host copy is pinned and reused once the previous

@@ -340,6 +340,10 @@ bz_status bz_create(const bz_config* cfg, bz_ctx** out)
     std::unique_ptr<bz_ctx> c(new (std::nothrow) bz_ctx());
     if (!c) return BZ_E_OOM;
     c->device = dev;
+    if (noalloc_scratch_address(dev, &c->noalloc_scratch) != hipSuccess) {  // bz_cull_noalloc's fixed scratch
+        (void)hipGetLastError();
+        return BZ_E_HIP;
+    }
     c->n_cu = (uint32_t)prop.multiProcessorCount;
     c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
     std::snprintf(c->device_name, sizeof c->device_name, "%s", prop.gcnArchName);

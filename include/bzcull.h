@@ -268,6 +268,18 @@ typedef struct bz_cull_args {
  * the caller as *count_dev > capacity. */
 BZ_API bz_status bz_cull(bz_ctx* ctx, const bz_cull_args* args, void* stream);
 
+/* bz_cull without any device allocation. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream or graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, passes, outputs and status codes as bz_cull; the kernels' compaction scratch is a fixed device array of
+ * 33 652 992 bytes that the library holds on every GPU it runs on until the process ends, instead of the per-stream
+ * buffer bz_cull grows. At most BZ_NOALLOC_MAX_OBJECTS objects per call (BZ_E_INVALID otherwise, nothing launched).
+ * BZ_CULL_FRAME_CACHE works once a bz_cull with the flag has allocated the scene's cache for the range (BZ_E_STATE
+ * before). Nothing in the array needs to start at zero: every call rewrites what it reads. All calls on one GPU share
+ * the array, so calls that overlap in time on one GPU (on different streams, or from several host threads) race;
+ * calls ordered on one stream do not. */
+#define BZ_NOALLOC_MAX_OBJECTS 4194304u
+BZ_API bz_status bz_cull_noalloc(bz_ctx* ctx, const bz_cull_args* args, void* stream);
+
 /* Cluster two-level cull (PreClusterDrawCull.comp.glsl:13-47 then InitialClusterCull.comp.glsl:12-55).
  * Object pass: frustum + LOD, as PreClusterDrawCull (with BZ_CLUSTER_OBJECT_HIZ in `flags`, also the Hi-Z test
  * against `pyramid`, which the reference's object pass does not have); expansion of the chosen LOD's clusters in
