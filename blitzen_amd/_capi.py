@@ -118,6 +118,7 @@ class SynthSceneCfg(C.Structure):
 OK, E_INVALID, E_OOM, E_HIP, E_CAPACITY, E_BUDGET, E_TIMEOUT, E_NOT_READY, E_NOT_FOUND, E_STATE, E_NODEVICE, \
     E_INTERNAL, E_CANCELLED = 0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11, -12
 PASS_EARLY, PASS_LATE, PASS_PLAIN, PASS_TEMPORAL, PASS_ONPC = 0, 1, 2, 3, 4
+DEFAULT_WAIT_LIMIT = 1048576
 CULL_EARLY_HIZ = 0x1
 CULL_FRAME_CACHE = 0x2
 CLUSTER_FRUSTUM, CLUSTER_HIZ, CLUSTER_CONE = 0x1, 0x2, 0x4
@@ -170,6 +171,7 @@ SIGNATURES = {
     "bz_cull_clusters": (_I32, [_VP, C.POINTER(ClusterCullArgs), _VP]),
     "bz_cull_batch": (_I32, [_VP, C.POINTER(ViewJob), _U32, _VP]),
     "bz_check": (_I32, [_VP, _VP]),
+    "bz_set_wait_limit": (_I32, [_VP, _U32]),
     "bz_get_tuning": (_I32, [_VP, _U32, C.POINTER(Tuning)]),
     "bz_set_tuning": (_I32, [_VP, _U32, C.POINTER(Tuning)]),
     "bz_autotune": (_I32, [_VP, _U32, C.POINTER(ViewJob), _U32, C.c_char_p, C.POINTER(Tuning), _VP]),

@@ -91,7 +91,7 @@ struct Pyramid : PyrPlan {
 
 struct StreamScratch {
     CompactScratch cs{};
-    uint32_t* err = nullptr;          // [0] spare, [1] pyramid tail wait gave up, [2] staged transform id out of range
+    uint32_t* err = nullptr;          // [0] instanced scan wait gave up, [1] pyramid tail wait gave up, [2] staged transform id out of range
     CullJob* jobs_dev = nullptr;
     CullJob* jobs_host = nullptr;     // pinned
     uint32_t jobs_cap = 0;
@@ -100,15 +100,20 @@ struct StreamScratch {
     uint32_t pjobs_cap = 0;
     hipEvent_t upload_done = nullptr; // last job-table upload from the pinned buffers
     ClusterScratch xs{};              // cluster pass
-    uint32_t* inst_hist = nullptr;    // LOD-instanced output: chunk x LOD histogram / list offsets
+    uint32_t* inst_hist = nullptr;    // LOD-instanced output: chunk x LOD histogram, then each chunk's first slots
     uint64_t inst_hist_cap = 0;
+    unsigned long long* inst_seg = nullptr;  // LOD-instanced output: the scan's published segment totals
+    uint64_t inst_seg_cap = 0;
+    uint32_t inst_epoch = 0;          // tag of the last LOD-instanced call on this stream
     ~StreamScratch();
 };
 
 struct Graph {
     hipGraphExec_t exec = nullptr;
     CompactScratch cs{};
-    uint32_t* err = nullptr;
+    uint32_t* err = nullptr;          // as StreamScratch::err, for the graph's own kernels
+    hipStream_t last_stream = nullptr;  // the stream of the last launch (bz_check on it reads `err`)
+    bool launched = false;
     CullJob* jobs_dev = nullptr;
     PyrBuildJob* pjobs_dev = nullptr;
     ~Graph();
@@ -137,6 +142,7 @@ struct Ctx {
     int device = 0;
     uint32_t n_cu = 0;
     uint64_t hbm_bytes = 0;
+    uint32_t wait_limit = BZ_DEFAULT_WAIT_LIMIT;  // sleeps before a kernel's bounded wait gives up
     char device_name[64] = {};
     std::string last_error;
     std::mutex err_mu;

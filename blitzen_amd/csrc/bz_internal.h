@@ -70,6 +70,7 @@ struct PyrBuildJob {
     uint32_t tile_w, tile_h, tiles_x, tiles_y;  // single-pass tiling of level 0
     uint32_t tile_levels;                        // levels reduced inside a tile (>= 1: includes level 0)
     uint32_t wg_start;                           // batched: first workgroup of this job
+    uint32_t spin_limit;                         // sleeps before a wait for a tail word gives up
     unsigned long long* tail;                    // per job: one flagged word per tile, zero between builds
     uint32_t level_offset[16];
 };
@@ -92,9 +93,17 @@ struct FrameCacheDev {
 // survivors there.
 hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
                        uint32_t per_cu, hipStream_t st, const FrameCacheDev* cache = nullptr);
-// The decide kernel alone: per tile, the emitting objects' (objectId, LOD) in ascending objectId and their count.
+// LOD histogram per chunk of decided tiles, filled by the decide kernel of the LOD-instanced output: workgroup c
+// decides tiles [c * tiles_per_chunk, (c + 1) * tiles_per_chunk) in order, counts its emitting objects per LOD in LDS
+// (n_lods <= kInstLdsBins) and writes the counts to hist[c * n_lods + LOD].
+struct InstHistDev {
+    uint32_t* hist;
+    uint32_t tiles_per_chunk, n_lods;
+};
+// The decide kernel alone: per tile, the emitting objects' (objectId, LOD) in ascending objectId and their count
+// (PLAIN decisions), and with `ih` their LOD histogram per chunk of tiles.
 hipError_t launch_decide_only(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items,
-                              uint32_t n_cu, uint32_t per_cu, hipStream_t st);
+                              uint32_t n_cu, uint32_t per_cu, hipStream_t st, const InstHistDev* ih = nullptr);
 
 // LOD-instanced output from a decided job (tiles of tile_size entries in s): per LOD, the ascending object ids
 // (at most per_lod) at lod * per_lod, the counters {lod * per_lod, exact count}, and one 32-byte command per LOD
@@ -105,9 +114,26 @@ struct InstancedJob {
     uint32_t* counters;           // [n_lods] x 2
     uint32_t* cmds;               // [capacity] x 8
     uint32_t* count;
-    uint32_t n_lods, per_lod, capacity, n_tiles, tile_size, tiles_per_chunk, n_chunks, pad;
+    uint32_t n_lods, per_lod, capacity, n_tiles, tile_size, tiles_per_chunk, n_chunks;
+    uint32_t epoch;               // this call's tag on the scan's published segment totals (never 0)
+    uint32_t* err;                // [0] set when the scan gives up its bounded wait
+    uint32_t spin_limit;
 };
-hipError_t launch_instanced(const InstancedJob& job, const CompactScratch& s, uint32_t* hist, hipStream_t st);
+// The LOD-instanced output's device state: `hist`, the chunk x LOD histogram (n_chunks x n_lods words, every word
+// written by the decide kernel or the histogram kernel), and `seg`, inst_seg_words() 64-bit words in which the scan
+// publishes each segment of kInstSegChunks chunks' totals, tagged with the call's epoch (zero at allocation).
+constexpr uint32_t kInstSegChunks = 32;
+inline uint32_t inst_segments(uint32_t n_chunks) { return n_chunks ? (n_chunks + kInstSegChunks - 1) / kInstSegChunks : 1u; }
+inline uint64_t inst_seg_words(uint32_t n_chunks, uint32_t n_lods) { return (uint64_t)inst_segments(n_chunks) * n_lods; }
+// LODs the decide kernel can histogram in LDS (more: a separate histogram kernel reads the decided entries).
+constexpr uint32_t kInstLdsBins = 4096;
+// with_hist: the decide kernel already wrote the histogram (chunks = its workgroups' tile ranges).
+hipError_t launch_instanced(const InstancedJob& job, const CompactScratch& s, uint32_t* hist, unsigned long long* seg,
+                            bool with_hist, hipStream_t st);
+// The decide of the LOD-instanced output with its LDS histogram: the workgroup count and tiles per workgroup
+// (= chunk) it will use for `job`, and the launch.
+void decide_inst_chunks(const CullJob& job, uint32_t block, uint32_t items, uint32_t n_cu, uint32_t per_cu,
+                        uint32_t* tiles_per_chunk, uint32_t* n_chunks);
 
 // LATE pass from a frame cache recorded with the same tile size (block x items) over the same range. It runs over
 // super-tiles of kSuperTile objects and needs late_cached_tiles(n) counts and late_cached_entries(n) staging slots.

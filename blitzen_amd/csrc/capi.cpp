@@ -91,7 +91,7 @@ Pyramid::~Pyramid()
 StreamScratch::~StreamScratch()
 {
     void* ps[] = { cs.counts, cs.staging, jobs_dev, pjobs_dev, err, xs.ccounts, xs.rbase, xs.scount, xs.sprefix,
-                   xs.survivors, inst_hist };
+                   xs.survivors, inst_hist, inst_seg };
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (jobs_host) (void)hipHostFree(jobs_host);
@@ -792,6 +792,7 @@ PyrBuildJob PyrPlan::build_job(const float* depth, uint32_t w, uint32_t h, float
     j.tiles_y = tiles_y;
     j.tile_levels = tile_levels;
     j.tail = tail;
+    j.spin_limit = BZ_DEFAULT_WAIT_LIMIT;
     for (uint32_t L = 0; L < mips; ++L) j.level_offset[L] = level_offset[L];
     return j;
 }
@@ -803,7 +804,9 @@ bz_status Ctx::build_pyramid(Pyramid* p, const float* depth, uint32_t w, uint32_
     if (s) return s;
     hipError_t e = hipSuccess;
     if (p->single_pass) {
-        e = launch_pyramid(p->build_job(depth, w, h), sc->err + 1, st);
+        PyrBuildJob j = p->build_job(depth, w, h);
+        j.spin_limit = wait_limit;
+        e = launch_pyramid(j, sc->err + 1, st);
     } else {
         e = launch_pyramid_level(depth, w, h, p->texels, p->base_w, p->base_h, st);
         for (uint32_t L = 1; L < p->mips && e == hipSuccess; ++L)
@@ -843,6 +846,7 @@ bz_status bz_pyramid_build_batch(bz_ctx* ctx, const bz_pyramid_job* jobs, uint32
             continue;
         }
         PyrBuildJob j = p->build_job(jobs[k].depth_dev, jobs[k].w, jobs[k].h);
+        j.spin_limit = ctx->wait_limit;
         j.wg_start = wg;
         wg += j.tiles_x * j.tiles_y;
         sp.push_back(j);
@@ -1011,26 +1015,53 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
     StreamScratch* sc = nullptr;
     st_ = ctx->scratch_for(st, job.n_tiles, (uint64_t)job.n_tiles * t.block * t.items, &sc);
     if (st_) return st_;
+    // Chunks of decided tiles, each with a histogram row of n_lods counters. When the LODs fit the decide kernel's
+    // LDS histogram, a chunk is one decide workgroup's contiguous run of tiles (as many chunks as workgroups fit at
+    // once); otherwise the histogram kernel counts up to 512 chunks (at most ~16 M counters).
+    const bool fused = s->n_lods <= kInstLdsBins;
     if (job.n) {
-        // chunks of decided tiles: the histogram holds n_chunks x n_lods counters (at most ~16 M)
-        const uint32_t target = std::max<uint32_t>(1u, std::min<uint32_t>(512u, (1u << 24) / s->n_lods));
-        ij.tiles_per_chunk = (job.n_tiles + target - 1) / target;
-        ij.n_chunks = (job.n_tiles + ij.tiles_per_chunk - 1) / ij.tiles_per_chunk;
         ij.n_tiles = job.n_tiles;
-        const uint64_t need = (uint64_t)ij.n_chunks * s->n_lods;
+        if (fused) {
+            decide_inst_chunks(job, t.block, t.items, ctx->n_cu, t.grid_per_cu, &ij.tiles_per_chunk, &ij.n_chunks);
+        } else {
+            const uint32_t target = std::max<uint32_t>(1u, std::min<uint32_t>(512u, (1u << 24) / s->n_lods));
+            ij.tiles_per_chunk = (job.n_tiles + target - 1) / target;
+            ij.n_chunks = (job.n_tiles + ij.tiles_per_chunk - 1) / ij.tiles_per_chunk;
+        }
+    }
+    const uint64_t need = std::max<uint64_t>(1, (uint64_t)ij.n_chunks * s->n_lods);
+    const uint64_t need_seg = inst_seg_words(ij.n_chunks, s->n_lods);
+    if (sc->inst_hist_cap < need || sc->inst_seg_cap < need_seg) {
+        hipError_t e = hipStreamSynchronize(st);
         if (sc->inst_hist_cap < need) {
-            hipError_t e = hipStreamSynchronize(st);
             if (sc->inst_hist) (void)hipFree(sc->inst_hist);
             sc->inst_hist = nullptr;
             sc->inst_hist_cap = 0;
             if (e == hipSuccess) e = hipMalloc((void**)&sc->inst_hist, need * 4);
-            if (e != hipSuccess) return ctx->hip_fail(e, "instancing histogram");
-            sc->inst_hist_cap = need;
+            if (e == hipSuccess) sc->inst_hist_cap = need;
         }
-        hipError_t e = launch_decide_only(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st);
+        if (e == hipSuccess && sc->inst_seg_cap < need_seg) {
+            // the scan's published totals: zero = the tag of no call (epochs start at 1)
+            if (sc->inst_seg) (void)hipFree(sc->inst_seg);
+            sc->inst_seg = nullptr;
+            sc->inst_seg_cap = 0;
+            e = hipMalloc((void**)&sc->inst_seg, need_seg * 8);
+            if (e == hipSuccess) e = hipMemset(sc->inst_seg, 0, need_seg * 8);
+            if (e == hipSuccess) sc->inst_seg_cap = need_seg;
+        }
+        if (e != hipSuccess) return ctx->hip_fail(e, "instancing scratch");
+    }
+    if (++sc->inst_epoch == 0) ++sc->inst_epoch;
+    ij.epoch = sc->inst_epoch;
+    ij.err = sc->err;
+    ij.spin_limit = ctx->wait_limit;
+    if (job.n) {
+        const InstHistDev ih{ sc->inst_hist, ij.tiles_per_chunk, ij.n_lods };
+        hipError_t e = launch_decide_only(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st,
+                                          fused ? &ih : nullptr);
         if (e != hipSuccess) return ctx->hip_fail(e, "decide launch");
     }
-    hipError_t e = launch_instanced(ij, sc->cs, sc->inst_hist, st);
+    hipError_t e = launch_instanced(ij, sc->cs, sc->inst_hist, sc->inst_seg, fused, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "instanced launch");
     return BZ_OK;
 }
@@ -1220,17 +1251,44 @@ bz_status bz_check(bz_ctx* ctx, void* stream)
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
-    auto it = ctx->scratch.find((uintptr_t)st);
-    if (it == ctx->scratch.end() || !it->second->err) return BZ_OK;
+    // the flags of the stream's own launches and of the graphs last launched on it, each cleared once read
     uint32_t flags[4] = { 0, 0, 0, 0 };
-    e = hipMemcpy(flags, it->second->err, sizeof flags, hipMemcpyDeviceToHost);
+    auto read_flags = [&](uint32_t* err) -> hipError_t {
+        uint32_t f[4] = { 0, 0, 0, 0 };
+        hipError_t r = hipMemcpy(f, err, sizeof f, hipMemcpyDeviceToHost);
+        if (r != hipSuccess) return r;
+        if (f[0] | f[1] | f[2] | f[3]) r = hipMemset(err, 0, sizeof f);
+        for (int i = 0; i < 4; ++i) flags[i] |= f[i];
+        return r;
+    };
+    auto it = ctx->scratch.find((uintptr_t)st);
+    if (it != ctx->scratch.end() && it->second->err) e = read_flags(it->second->err);
+    for (auto& g : ctx->graphs)
+        if (e == hipSuccess && g.second->launched && g.second->last_stream == st) e = read_flags(g.second->err);
     if (e != hipSuccess) return ctx->hip_fail(e, "error flag read");
+    if (flags[1]) {
+        // A tail workgroup that gave up may have missed hand-off words that tiles stored after it looked: clear every
+        // pyramid's words (with the device idle, so no build is in flight) for the next builds to start from zero.
+        e = hipDeviceSynchronize();
+        for (auto& p : ctx->pyramids)
+            if (e == hipSuccess && p.second->tail)
+                e = hipMemset(p.second->tail, 0, (size_t)p.second->tiles_x * p.second->tiles_y * 8u);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) return ctx->hip_fail(e, "pyramid hand-off reset");
+    }
     if (flags[0] || flags[1] || flags[2]) {
-        (void)hipMemset(it->second->err, 0, 16);
         if (flags[2] && !flags[0] && !flags[1])
             return ctx->fail(BZ_E_INVALID, "a staged transform update named a transform id out of range (skipped)");
-        return ctx->fail(BZ_E_INTERNAL, "a kernel gave up a bounded wait");
+        return ctx->fail(BZ_E_INTERNAL, "a kernel gave up a bounded wait (the pyramid built then holds wrong values "
+                                        "until rebuilt)");
     }
+    return BZ_OK;
+}
+
+bz_status bz_set_wait_limit(bz_ctx* ctx, uint32_t sleeps)
+{
+    if (!ctx) return BZ_E_INVALID;
+    ctx->wait_limit = sleeps;
     return BZ_OK;
 }
 
@@ -1261,6 +1319,7 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
             continue;
         }
         PyrBuildJob j = p->build_job(pyr_jobs[k].depth_dev, pyr_jobs[k].w, pyr_jobs[k].h);
+        j.spin_limit = ctx->wait_limit;
         j.wg_start = wg;
         wg += j.tiles_x * j.tiles_y;
         pj.push_back(j);
@@ -1337,6 +1396,8 @@ bz_status bz_graph_launch(bz_ctx* ctx, bz_graph graph, void* stream)
     if (it == ctx->graphs.end()) return ctx->fail(BZ_E_INVALID, "unknown graph handle");
     hipError_t e = hipGraphLaunch(it->second->exec, (hipStream_t)stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "hipGraphLaunch");
+    it->second->last_stream = (hipStream_t)stream;
+    it->second->launched = true;
     return BZ_OK;
 }
 

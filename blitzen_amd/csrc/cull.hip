@@ -218,11 +218,13 @@ struct NoPrefetch {
 // flight through the rest of the tile (vmcnt retires in issue order: nothing later waits on them).
 // CL (cluster object pass): ccounts[gtile] also receives the sum of the emitting objects' LOD cluster counts.
 // CACHE (EARLY with all streams loaded): every frustum survivor of the tile is recorded in the frame cache `fc`.
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, bool CL = false, bool CACHE = false, class Prefetch>
+// INST (LOD-instanced output): each emitting object also adds 1 to its LOD's bin of the workgroup's LDS histogram.
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool LDS_TAB, bool CL = false, bool CACHE = false, bool INST = false,
+          class Prefetch>
 __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, const Streams<ITEMS>& s,
                                              uint32_t* counts, uint2* staging, uint32_t* ccounts,
                                              TileSmem<BLOCK, ITEMS>& sm, Prefetch&& prefetch,
-                                             FrameCacheDev fc = FrameCacheDev{})
+                                             FrameCacheDev fc = FrameCacheDev{}, uint32_t* lds_hist = nullptr)
 {
     constexpr bool lds_tab = LDS_TAB;
     constexpr int NW = BLOCK / 64;
@@ -351,6 +353,12 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
         wave_lds_sync();
     }
 
+    if constexpr (INST) {  // the LOD histogram of the workgroup's tiles (LDS)
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+            if (emit[k]) atomicAdd(&lds_hist[lod_abs[k]], 1u);
+    }
+
     if constexpr (CL) {  // the tile's cluster total, for the expansion's record offsets
         uint32_t cc = 0;
 #pragma unroll
@@ -441,39 +449,59 @@ __device__ __forceinline__ void decide_tile(const CullJob& J, uint32_t tile_in_j
 // Persistent decide for one view: workgroup b takes tiles b, b + G, b + 2G, ... (no workgroup waits on another, so
 // any residency is safe). Tables are staged once; the next tile's streams are issued while the current one is
 // decided (two register sets, alternating, so no register copy waits on a load in flight).
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false>
-__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging, FrameCacheDev fc)
+// INST: workgroup b takes the contiguous tiles [b * tpc, (b + 1) * tpc) instead (tpc = ih.tiles_per_chunk), counts
+// its emitting objects per LOD in LDS and writes the counts to row b of ih.hist.
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false, bool INST = false>
+__global__ __launch_bounds__(BLOCK) void decide_kernel(CullJob J, uint32_t* counts, uint2* staging, FrameCacheDev fc,
+                                                       InstHistDev ih)
 {
     __shared__ TileSmem<BLOCK, ITEMS> sm;
+    __shared__ uint32_t s_hist[INST ? kInstLdsBins : 1];
     constexpr bool lds_tab = LDS_TAB;
     constexpr bool NT = (MODE == MODE_EARLY && CACHE) || (MODE == MODE_OTHER && !HIZ);
-    const uint32_t G = gridDim.x;
-    uint32_t t = blockIdx.x;
+    uint32_t t, t_end, step;
+    if constexpr (INST) {
+        t = blockIdx.x * ih.tiles_per_chunk;
+        t_end = t + ih.tiles_per_chunk < J.n_tiles ? t + ih.tiles_per_chunk : J.n_tiles;
+        step = 1;
+        for (uint32_t i = threadIdx.x; i < ih.n_lods; i += BLOCK) s_hist[i] = 0u;  // (ordered by the barrier below)
+    } else {
+        t = blockIdx.x;
+        t_end = J.n_tiles;
+        step = gridDim.x;
+    }
     Streams<ITEMS> A, B;
     issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, t, A);
     stage_tables<BLOCK, ITEMS>(J, sm, lds_tab);
     __syncthreads();
     while (true) {
         {
-            const uint32_t tn = t + G;
+            const uint32_t tn = t + step;
             // past the last tile, tile n_tiles is issued: every lane is out of range and clamped to the job's last
             // object, so each stream costs one cache line (discarded). A branch around the loads instead was 2.7 us
             // slower per pass, re-issuing the current tile 2.2 us slower (with non-temporal loads its lines are
             // fetched again).
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, A, counts, staging, nullptr, sm, [&]() {
-                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < J.n_tiles ? tn : J.n_tiles, B);
-            }, fc);
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE, INST>(J, t, A, counts, staging, nullptr, sm,
+                                                                                [&]() {
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < t_end ? tn : J.n_tiles, B);
+            }, fc, s_hist);
             t = tn;
-            if (t >= J.n_tiles) break;
+            if (t >= t_end) break;
         }
         {
-            const uint32_t tn = t + G;
-            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE>(J, t, B, counts, staging, nullptr, sm, [&]() {
-                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < J.n_tiles ? tn : J.n_tiles, A);
-            }, fc);
+            const uint32_t tn = t + step;
+            process_tile<BLOCK, ITEMS, MODE, HIZ, LDS_TAB, false, CACHE, INST>(J, t, B, counts, staging, nullptr, sm,
+                                                                                [&]() {
+                issue_tile<BLOCK, ITEMS, MODE, XF_ID, CACHE, NT>(J, tn < t_end ? tn : J.n_tiles, A);
+            }, fc, s_hist);
             t = tn;
-            if (t >= J.n_tiles) break;
+            if (t >= t_end) break;
         }
+    }
+    if constexpr (INST) {
+        __syncthreads();
+        uint32_t* row = ih.hist + (size_t)blockIdx.x * ih.n_lods;
+        for (uint32_t i = threadIdx.x; i < ih.n_lods; i += BLOCK) row[i] = s_hist[i];
     }
 }
 
@@ -885,15 +913,15 @@ hipError_t launch_late_cached_t(const CullJob& job, const CompactScratch& s, con
 }
 
 // Persistent grid: `per_cu` workgroups per CU, or (per_cu == 0) as many as are resident at once per CU.
-template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false>
+template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool XF_ID, bool LDS_TAB, bool CACHE = false, bool INST = false>
 void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t* counts, uint2* staging, hipStream_t st,
-                   const FrameCacheDev& fc = FrameCacheDev{})
+                   const FrameCacheDev& fc = FrameCacheDev{}, const InstHistDev& ih = InstHistDev{})
 {
     static int resident = 0;
     if (resident == 0) {
         int r = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE>, BLOCK, 0) != hipSuccess ||
+                &r, decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE, INST>, BLOCK, 0) != hipSuccess ||
             r < 1)
             r = 1;
         resident = r;
@@ -901,8 +929,46 @@ void launch_decide(const CullJob& job, uint32_t n_cu, uint32_t per_cu, uint32_t*
     uint64_t grid = (uint64_t)n_cu * (per_cu ? per_cu : (uint32_t)resident);
     if (grid > job.n_tiles) grid = job.n_tiles;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE>), dim3((uint32_t)grid),
-                       dim3(BLOCK), 0, st, job, counts, staging, fc);
+    if (INST) grid = (job.n_tiles + ih.tiles_per_chunk - 1) / ih.tiles_per_chunk;  // one chunk per workgroup
+    hipLaunchKernelGGL((decide_kernel<BLOCK, ITEMS, MODE, HIZ, XF_ID, LDS_TAB, CACHE, INST>), dim3((uint32_t)grid),
+                       dim3(BLOCK), 0, st, job, counts, staging, fc, ih);
+}
+
+// Workgroups per CU of the instanced decide (as launch_decide counts them for the persistent grid).
+template <int BLOCK, int ITEMS>
+uint32_t inst_resident()
+{
+    static int resident = 0;
+    if (resident == 0) {
+        int r = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &r, decide_kernel<BLOCK, ITEMS, MODE_OTHER, false, true, true, false, true>, BLOCK, 0) != hipSuccess ||
+            r < 1)
+            r = 1;
+        resident = r;
+    }
+    return (uint32_t)resident;
+}
+
+// The decide of the LOD-instanced output: PLAIN decisions plus the chunks' LOD histograms.
+template <int BLOCK, int ITEMS>
+hipError_t launch_decide_inst_t(const CullJob& job, const CompactScratch& s, uint32_t n_cu, uint32_t per_cu,
+                                const InstHistDev& ih, hipStream_t st)
+{
+    const bool lds = job.scene.n_surfaces <= kLdsSurfaces && job.scene.n_lods <= kLdsLods;
+    const FrameCacheDev none{};
+    if (job.scene.obj_xf == nullptr) {
+        if (lds) launch_decide<BLOCK, ITEMS, MODE_OTHER, false, true, true, false, true>(job, n_cu, per_cu, s.counts,
+                                                                                      s.staging, st, none, ih);
+        else launch_decide<BLOCK, ITEMS, MODE_OTHER, false, true, false, false, true>(job, n_cu, per_cu, s.counts,
+                                                                                   s.staging, st, none, ih);
+    } else {
+        if (lds) launch_decide<BLOCK, ITEMS, MODE_OTHER, false, false, true, false, true>(job, n_cu, per_cu, s.counts,
+                                                                                       s.staging, st, none, ih);
+        else launch_decide<BLOCK, ITEMS, MODE_OTHER, false, false, false, false, true>(job, n_cu, per_cu, s.counts,
+                                                                                    s.staging, st, none, ih);
+    }
+    return hipGetLastError();
 }
 
 template <int BLOCK, int ITEMS, int MODE, bool HIZ, bool CACHE = false>
@@ -979,13 +1045,30 @@ hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t blo
 }
 
 hipError_t launch_decide_only(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items,
-                              uint32_t n_cu, uint32_t per_cu, hipStream_t st)
+                              uint32_t n_cu, uint32_t per_cu, hipStream_t st, const InstHistDev* ih)
 {
-#define X(B, I) \
-    if (block == B && items == I) return launch_cull_t<B, I>(job, s, n_cu, per_cu, st, nullptr, false);
+#define X(B, I)                                                                                                      \
+    if (block == B && items == I)                                                                                     \
+        return ih ? launch_decide_inst_t<B, I>(job, s, n_cu, per_cu, *ih, st)                                         \
+                  : launch_cull_t<B, I>(job, s, n_cu, per_cu, st, nullptr, false);
     BZ_TUNING_TABLE(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+void decide_inst_chunks(const CullJob& job, uint32_t block, uint32_t items, uint32_t n_cu, uint32_t per_cu,
+                        uint32_t* tiles_per_chunk, uint32_t* n_chunks)
+{
+    uint32_t resident = 1;
+#define X(B, I) \
+    if (block == B && items == I) resident = inst_resident<B, I>();
+    BZ_TUNING_TABLE(X)
+#undef X
+    // as many chunks as workgroups fit at once: one round, each workgroup a contiguous run of tiles
+    const uint64_t slots = (uint64_t)n_cu * (per_cu ? per_cu : resident);
+    const uint64_t tpc = slots ? (job.n_tiles + slots - 1) / slots : job.n_tiles;
+    *tiles_per_chunk = (uint32_t)(tpc ? tpc : 1);
+    *n_chunks = (job.n_tiles + *tiles_per_chunk - 1) / *tiles_per_chunk;
 }
 
 hipError_t launch_late_cached(const CullJob& job_in, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,

@@ -5,10 +5,13 @@
 // of its LOD, and each LOD with instances gets one instanced DrawIndexed command. The reference appends with
 // InterlockedAdd (nondeterministic order, and a full list overflows into the next LOD's); here each list holds its
 // objects in ascending objectId, at most per_lod of them, and the counter keeps the exact count:
-//   hist:    one workgroup per chunk of decided tiles x LOD slice: an LDS histogram of the chunk's LODs (the
-//            chunk's entries loaded as 64-entry groups, kBatch groups per round trip);
-//   scan:    one workgroup per 64 LODs: exclusive prefix of the histogram over chunks (the chunk's first slot in the
-//            LOD's list) and the counter {lod * per_lod, total};
+//   hist:    per chunk of decided tiles, the emitting objects' count per LOD: counted by the decide kernel itself in
+//            LDS when the LODs fit kInstLdsBins (a chunk is then one workgroup's contiguous run of tiles, and the
+//            decided entries are read once, by the scatter); otherwise one workgroup per chunk x LOD slice builds an
+//            LDS histogram from the decided entries (loaded as 64-entry groups, kBatch groups per round trip);
+//   scan:    one workgroup per 64 LODs x segment of 32 chunks: exclusive prefix of the histogram over the chunks (the
+//            chunk's first slot in the LOD's list; the segments' totals pass between workgroups by look-back) and the
+//            counter {lod * per_lod, total};
 //   cmds:    one extra workgroup of the scatter launch: the LODs with instances in ascending order -> 32-byte commands
 //            and their number;
 //   scatter: one workgroup per chunk x LOD slice: per-wave LOD counts over quarters of the chunk's entries, a hand-over
@@ -20,7 +23,7 @@ namespace bz {
 
 namespace {
 
-constexpr uint32_t kSlice = 8192;  // LOD bins per histogram / scatter workgroup (LDS counters)
+constexpr uint32_t kSlice = 8192;  // LOD bins per histogram workgroup (LDS counters)
 constexpr uint32_t kBatch = 8;     // 64-entry groups loaded per round trip
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
@@ -95,46 +98,79 @@ __global__ __launch_bounds__(256) void inst_hist_kernel(InstancedJob J, const ui
     for (uint32_t i = tid; i < l1 - l0; i += 256) hist[(size_t)c * J.n_lods + l0 + i] = h[i];
 }
 
-// One workgroup per 64 LODs (one LOD per lane): wave w holds chunks [w * seg, (w + 1) * seg) of its lane's LOD in
-// registers (every load issued at once), the 16 segment sums are scanned across the waves in LDS, and each chunk's
-// count is replaced by its first slot in the LOD's list; wave 0 writes the counters {lod * per_lod, total}.
-constexpr uint32_t kScanWaves = 16, kScanSeg = 32;  // up to kScanWaves * kScanSeg = 512 chunks
+// Scan of the chunk x LOD histogram over the chunks, per LOD, by workgroups of (64 LODs: one per lane) x (segment of
+// kSegChunks chunks: kSegChunks / 4 consecutive chunks per wave), so that a scan of ~1 M counters has hundreds of
+// workgroups. A workgroup publishes its segment's total per LOD in one 64-bit word tagged with the call's epoch
+// (value and tag in one atomic store: no fence, no reset), and sums the totals of the LOD group's earlier segments
+// (look-back over at most a few dozen words, all loaded at once; a word of an earlier segment whose workgroup has not
+// stored it yet is polled, with a bounded wait). Workgroups are dispatched in order of their linear index, segment
+// major, so a workgroup waits only on workgroups dispatched before it. Each chunk's count is replaced by its first
+// slot in the LOD's list; the last segment writes the counters {lod * per_lod, total}.
+constexpr uint32_t kSegChunks = kInstSegChunks, kSegPerWave = kSegChunks / 4;
 
-__global__ __launch_bounds__(kScanWaves * 64) void inst_scan_kernel(InstancedJob J, uint32_t* hist)
+__global__ __launch_bounds__(256) void inst_scan_kernel(InstancedJob J, uint32_t* hist, unsigned long long* seg)
 {
-    __shared__ uint32_t seg_sum[kScanWaves][64];
+    __shared__ uint32_t wsum[4][64];
+    __shared__ uint32_t before_s[64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t l = blockIdx.x * 64u + lane;
+    const uint32_t l = blockIdx.x * 64u + lane, s = blockIdx.y, n_seg = gridDim.y;
     const bool ok = l < J.n_lods;
-    const uint32_t segn = (J.n_chunks + kScanWaves - 1u) / kScanWaves;
-    const uint32_t c0 = wave * segn;
-    uint32_t v[kScanSeg];
+    const uint32_t c0 = s * kSegChunks + wave * kSegPerWave;
+    uint32_t v[kSegPerWave];
 #pragma unroll
-    for (uint32_t j = 0; j < kScanSeg; ++j) {
-        const uint32_t c = c0 + j;
-        v[j] = (ok && j < segn && c < J.n_chunks) ? hist[(size_t)c * J.n_lods + l] : 0u;
-    }
+    for (uint32_t j = 0; j < kSegPerWave; ++j)
+        v[j] = (ok && c0 + j < J.n_chunks) ? hist[(size_t)(c0 + j) * J.n_lods + l] : 0u;
     uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kScanSeg; ++j) sum += v[j];
-    seg_sum[wave][lane] = sum;
+    for (uint32_t j = 0; j < kSegPerWave; ++j) sum += v[j];
+    wsum[wave][lane] = sum;
     __syncthreads();
     uint32_t run = 0, total = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kScanWaves; ++w) {
-        const uint32_t x = seg_sum[w][lane];
+    for (uint32_t w = 0; w < 4; ++w) {
+        const uint32_t x = wsum[w][lane];
         run += w < wave ? x : 0u;
         total += x;
     }
+    const unsigned long long tag = (unsigned long long)J.epoch << 32;
+    if (wave == 0) {
+        if (ok && s + 1u < n_seg)
+            __hip_atomic_store(&seg[(size_t)s * J.n_lods + l], tag | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t before = 0;
+        for (uint32_t k0 = 0; k0 < s; k0 += 16) {
+            unsigned long long w[16];
 #pragma unroll
-    for (uint32_t j = 0; j < kScanSeg; ++j) {
-        const uint32_t c = c0 + j;
-        if (ok && j < segn && c < J.n_chunks) hist[(size_t)c * J.n_lods + l] = run;
-        run += v[j];
+            for (uint32_t k = 0; k < 16; ++k)
+                w[k] = (ok && k0 + k < s) ? __hip_atomic_load(&seg[(size_t)(k0 + k) * J.n_lods + l], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : tag;
+#pragma unroll
+            for (uint32_t k = 0; k < 16; ++k) {
+                uint32_t spins = 0;
+                while ((w[k] & 0xffffffff00000000ull) != tag) {
+                    if (++spins > J.spin_limit) {
+                        atomicExch(J.err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    w[k] = __hip_atomic_load(&seg[(size_t)(k0 + k) * J.n_lods + l], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                }
+                before += (uint32_t)w[k];
+            }
+        }
+        before_s[lane] = before;
+        if (ok && s + 1u == n_seg) {
+            J.counters[2u * l] = l * J.per_lod;
+            J.counters[2u * l + 1u] = before + total;
+        }
     }
-    if (wave == 0 && ok) {
-        J.counters[2u * l] = l * J.per_lod;
-        J.counters[2u * l + 1u] = total;
+    __syncthreads();
+    run += before_s[lane];
+#pragma unroll
+    for (uint32_t j = 0; j < kSegPerWave; ++j) {
+        if (ok && c0 + j < J.n_chunks) hist[(size_t)(c0 + j) * J.n_lods + l] = run;
+        run += v[j];
     }
 }
 
@@ -212,7 +248,8 @@ constexpr uint32_t kScatterSlice = 2048;
 constexpr uint32_t kScatterWaves = 4;
 
 __global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(InstancedJob J, const uint32_t* counts,
-                                                                          const uint2* staging, const uint32_t* base)
+                                                                          const uint2* staging,
+                                                                          const uint32_t* hist)
 {
     __shared__ uint32_t slot[kScatterWaves][kScatterSlice];  // per wave: LOD -> count (1), then next slot (2, 3)
     __shared__ uint32_t running[kScatterSlice];              // per LOD: the next slot after the windows so far
@@ -228,7 +265,7 @@ __global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(Instan
     const uint32_t nbits = nl > 1u ? 32u - (uint32_t)__clz(nl - 1u) : 0u;  // bits of the slice's largest key
     const unsigned long long below = (1ull << lane) - 1ull;
     {
-        const uint32_t* row = base + (size_t)c * J.n_lods + l0;
+        const uint32_t* row = hist + (size_t)c * J.n_lods + l0;
         uint32_t v[kScatterSlice / (kScatterWaves * 64)];
 #pragma unroll
         for (uint32_t k = 0; k < kScatterSlice / (kScatterWaves * 64); ++k) {
@@ -313,17 +350,18 @@ __global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(Instan
 
 }  // namespace
 
-hipError_t launch_instanced(const InstancedJob& J, const CompactScratch& s, uint32_t* hist, hipStream_t st)
+hipError_t launch_instanced(const InstancedJob& J, const CompactScratch& s, uint32_t* hist, unsigned long long* seg,
+                            bool with_hist, hipStream_t st)
 {
-    const uint32_t slices = (J.n_lods + kSlice - 1) / kSlice;
-    if (J.n_chunks > kScanWaves * kScanSeg) return hipErrorInvalidValue;  // the scan holds at most 512 chunks
-    if (J.n_chunks) {
+    if (J.n_chunks && !with_hist) {
+        const uint32_t slices = (J.n_lods + kSlice - 1) / kSlice;
         hipLaunchKernelGGL(inst_hist_kernel, dim3(J.n_chunks, slices), dim3(256), 0, st, J,
                            (const uint32_t*)s.counts, (const uint2*)s.staging, hist);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(inst_scan_kernel, dim3((J.n_lods + 63) / 64), dim3(kScanWaves * 64), 0, st, J, hist);
+    hipLaunchKernelGGL(inst_scan_kernel, dim3((J.n_lods + 63) / 64, inst_segments(J.n_chunks)), dim3(256), 0, st, J,
+                       hist, seg);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (!J.n_chunks) {

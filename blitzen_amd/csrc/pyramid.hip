@@ -19,7 +19,6 @@ namespace {
 constexpr uint32_t kPyrBlock = 256;
 constexpr uint32_t kTile = 32;
 constexpr unsigned long long kTailFlag = 1ull << 63;
-constexpr uint32_t kTailSpinLimit = 1u << 20;
 constexpr uint32_t kLdsJobs = 1024;  // batched: job start table staged in LDS up to this many jobs
 
 __device__ __forceinline__ float min4(float a, float b, float c, float d) { return glsl_min(glsl_min(a, b), glsl_min(c, d)); }
@@ -48,7 +47,7 @@ __device__ __forceinline__ float tail_finish(const PyrBuildJob& J, uint32_t idx,
 {
     uint32_t spins = 0;
     while (!(w & kTailFlag)) {
-        if (++spins > kTailSpinLimit) {
+        if (++spins > J.spin_limit) {
             atomicExch(err, 1u);
             break;
         }
@@ -68,7 +67,7 @@ __device__ __forceinline__ float tail_word(const PyrBuildJob& J, uint32_t idx, u
     unsigned long long w = __hip_atomic_load(&J.tail[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
     while (!(w & kTailFlag)) {
-        if (++spins > kTailSpinLimit) {
+        if (++spins > J.spin_limit) {
             atomicExch(err, 1u);
             break;
         }

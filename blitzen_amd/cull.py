@@ -116,6 +116,10 @@ class Context:
     def check_kernels(self, stream=None):
         self.check(lib().bz_check(self.handle, stream), "bz_check")
 
+    def set_wait_limit(self, sleeps: int):
+        """Sleeps a kernel's bounded wait may take before giving up (pyramid builds launched afterwards)."""
+        self.check(lib().bz_set_wait_limit(self.handle, sleeps), "bz_set_wait_limit")
+
     def stream(self):
         s = C.c_void_p()
         self.check(lib().bz_stream_create(self.handle, C.byref(s)), "bz_stream_create")

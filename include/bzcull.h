@@ -372,8 +372,18 @@ typedef struct bz_tuning {
  * batch with bz_batch_graph_create instead. Jobs that write one visibility buffer in overlapping ranges race. */
 BZ_API bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, void* stream);
 /* Synchronizes `stream` and reports BZ_E_INTERNAL if a kernel launched on it gave up a bounded wait
- * since the last check (the flag is then cleared). */
+ * since the last check (the flag is then cleared). A pyramid build whose tail workgroup gave up waiting for a
+ * tile's hand-off word can leave such words set by tiles that finished later: the check then also waits for the
+ * device and clears the hand-off words of every pyramid of the context, so the next build of each is correct
+ * (the pyramid that was being built holds wrong values until rebuilt). */
 BZ_API bz_status bz_check(bz_ctx* ctx, void* stream);
+/* The number of short sleeps (s_sleep 1, about 64 clocks each) a bounded wait inside a kernel may take before it
+ * gives up; 0 gives up at the first word not yet handed over. Applies to pyramid builds launched afterwards through
+ * this context (bz_pyramid_build, bz_pyramid_build_batch, bz_batch_graph_create). Default BZ_DEFAULT_WAIT_LIMIT.
+ * A small limit forces the give-up path for tests of the error handling; with the default the waits end in a few
+ * microseconds. */
+#define BZ_DEFAULT_WAIT_LIMIT 1048576u
+BZ_API bz_status bz_set_wait_limit(bz_ctx* ctx, uint32_t sleeps);
 /* Tuning used by bz_cull (kernel 0) and bz_cull_batch (kernel 1); defaults are built in, bz_autotune
  * replaces them. */
 BZ_API bz_status bz_get_tuning(bz_ctx* ctx, uint32_t kernel /* 0 = single view, 1 = batched */, bz_tuning* out);
