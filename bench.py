@@ -334,6 +334,50 @@ def bench_next_rows(ctx, stream, scenes, view, transforms, steps: int):
     return out
 
 
+def bench_size_scaling(ctx, stream, view, steps: int, copies: int = 2):
+    """The PLAIN pass of one job at 4, 8 and 16 M objects (ranges [0, n) of two resident 16 M-object scenes, used
+    in turn so that a range is not cache-resident between launches). The pass is decide + emit; a job of more than
+    4096 tiles also scans its tile counts, so that every emit workgroup reads its output offset instead of re-summing
+    the earlier tiles' counts. Time per million objects stays flat when the emit is linear in the object count."""
+    import numpy as np
+
+    from blitzen_amd import _capi as A
+    from blitzen_amd import cull as bz
+    from blitzen_amd.hip import Event
+
+    n_max = 16 * 1048576
+    sc = bz.synth_scene(SEED + 16, n_max, N_SURFACES, N_LODS)
+    scenes = [ctx.scene_create(sc.objects, sc.transforms, sc.surfaces, sc.lods) for _ in range(copies)]
+    del sc
+    cmds, cnt = ctx.alloc(n_max // 2 * CMD_BYTES), ctx.alloc(4)
+    out = {"workload": "PLAIN pass, one job of n objects (C1 distributions), rotating over "
+                       f"{copies} resident 16 M-object scenes", "sizes": []}
+    for n in (4 * 1048576, 8 * 1048576, n_max):
+        def fn(k):
+            ctx.cull(scenes[k % copies], view, A.PASS_PLAIN, cmds, cnt, capacity=n_max // 2, n=n, stream=stream)
+        for k in range(2 * copies):
+            fn(k)
+        ctx.sync(stream)
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for k in range(steps):
+            fn(k)
+        e1.record(stream)
+        ctx.sync(stream)
+        ctx.check_kernels(stream)
+        us = e0.elapsed_ms(e1) / steps * 1e3
+        n_cmd = int(cnt.download(np.uint32, 1, stream)[0])
+        alg = n * 40 + n_cmd * CMD_BYTES + 4
+        out["sizes"].append({"objects": n, "tiles": n // 1024, "scanned_prefix": n // 1024 > 4096,
+                             "us": round(us, 2), "us_per_mobject": round(us / (n / 1048576), 3), "commands": n_cmd,
+                             "achieved_gbs": round(alg / (us * 1e-6) / 1e9, 1)})
+    for h in scenes:
+        ctx.scene_destroy(h)
+    cmds.free()
+    cnt.free()
+    return out
+
+
 def load_traffic():
     p = ROOT / "profiles" / "traffic_frame_c1.json"
     if p.exists():
@@ -355,6 +399,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-clusters", action="store_true")
+    ap.add_argument("--no-scaling", action="store_true", help="skip the PLAIN pass at 4, 8 and 16 M objects")
     ap.add_argument("--no-next", action="store_true", help="skip the PLAIN / ONPC / instanced / transform-update leg")
     ap.add_argument("--served-seconds", type=float, default=0.5)
     ap.add_argument("--no-frame-cache", action="store_true",
@@ -532,6 +577,9 @@ def main():
         log("bench: next rows (plain, onpc, instanced, transform updates) ...")
         result["next_rows"] = bench_next_rows(ctx, stream, [c[0] for c in copies], v, sc.transforms,
                                               max(5, min(args.steps, 20)))
+    if not args.no_scaling:
+        log("bench: size scaling (PLAIN pass, 4-16 M objects) ...")
+        result["size_scaling"] = bench_size_scaling(ctx, stream, v, max(5, min(args.steps, 20)))
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -739,8 +739,9 @@ bz_status bz_pyramid_create(bz_ctx* ctx, uint32_t base_w, uint32_t base_h, uint3
     if (!p) return BZ_E_OOM;
     if (!plan_pyramid(base_w, base_h, mips, p.get())) return ctx->fail(BZ_E_INVALID, "pyramid too large");
     const uint64_t off = p->n_texels;
-    hipError_t e = hipMalloc((void**)&p->texels, off * 4);
-    if (e == hipSuccess) e = hipMemset(p->texels, 0, off * 4);
+    // one spare texel past the last level: the Hi-Z lookup reads texels in pairs (footprint_texels)
+    hipError_t e = hipMalloc((void**)&p->texels, (off + 1) * 4);
+    if (e == hipSuccess) e = hipMemset(p->texels, 0, (off + 1) * 4);
     const size_t n_tiles = (size_t)p->tiles_x * p->tiles_y;
     if (e == hipSuccess) e = hipMalloc((void**)&p->tail, n_tiles * 8);
     if (e == hipSuccess) e = hipMemset(p->tail, 0, n_tiles * 8);

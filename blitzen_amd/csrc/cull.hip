@@ -284,9 +284,8 @@ __device__ __forceinline__ void process_tile(const CullJob& J, uint32_t gtile, c
             const uint32_t L = hiz_level(glsl_max(width, height), J.pyr.mips);
             const uint32_t W = level_dim(J.pyr.base_w, L), H = level_dim(J.pyr.base_h, L);
             const Footprint f = footprint((a0 + a2) * 0.5f, (a1 + a3) * 0.5f, W, H);
-            const float* lvl = J.pyr.texels + sm.lvl[L];
-            const float d00 = lvl[(size_t)f.y0 * W + f.x0], d01 = lvl[(size_t)f.y0 * W + f.x1];
-            const float d10 = lvl[(size_t)f.y1 * W + f.x0], d11 = lvl[(size_t)f.y1 * W + f.x1];
+            float d00, d01, d10, d11;
+            footprint_texels(J.pyr.texels + sm.lvl[L], W, f, d00, d01, d10, d11);
             __builtin_amdgcn_sched_barrier(0);  // keep the gathers ahead of the prefetch in issue order
             between();
             __builtin_amdgcn_sched_barrier(0);

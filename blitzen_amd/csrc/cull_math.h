@@ -159,6 +159,24 @@ __device__ __forceinline__ float footprint_min(const float* __restrict__ level, 
     return glsl_min(glsl_min(d00, d01), glsl_min(d10, d11));
 }
 
+// The four footprint texels of a pyramid level as two 8-byte loads, one per row: texels (xa, xa + 1) with xa = x0,
+// or x0 - 1 when x0 is the row's last texel, so that x0 and x1 (each x0 or x0 + 1, clamped) are both in the pair.
+// On a level one texel wide the pair's second word is the texel after the row (the next row or level, or the spare
+// texel every pyramid allocation holds past its last level) and is not used. The values are the ones four single
+// loads return; the gathers take half the address-unit cycles.
+__device__ __forceinline__ void footprint_texels(const float* __restrict__ level, uint32_t W, const Footprint& f,
+                                                 float& d00, float& d01, float& d10, float& d11)
+{
+    const uint32_t xa = (f.x0 + 1u < W || f.x0 == 0u) ? f.x0 : f.x0 - 1u;
+    float2 r0, r1;
+    __builtin_memcpy(&r0, level + (size_t)f.y0 * W + xa, 8);
+    __builtin_memcpy(&r1, level + (size_t)f.y1 * W + xa, 8);
+    d00 = f.x0 == xa ? r0.x : r0.y;
+    d01 = f.x1 == xa ? r0.x : r0.y;
+    d10 = f.x0 == xa ? r1.x : r1.y;
+    d11 = f.x1 == xa ? r1.x : r1.y;
+}
+
 // level_offset: the pyramid's per-level texel offsets (kept in LDS by the caller).
 __device__ __forceinline__ bool occlusion_visible(const ViewConsts& v, const PyramidDesc& p, const uint32_t* level_offset,
                                                   float a0, float a1, float a2, float a3, float cz, float r)
@@ -170,7 +188,9 @@ __device__ __forceinline__ bool occlusion_visible(const ViewConsts& v, const Pyr
     const float w = (a1 + a3) * 0.5f;
     const uint32_t W = level_dim(p.base_w, L), H = level_dim(p.base_h, L);
     const Footprint f = footprint(u, w, W, H);
-    const float depth = footprint_min(p.texels + level_offset[L], W, f);
+    float d00, d01, d10, d11;
+    footprint_texels(p.texels + level_offset[L], W, f, d00, d01, d10, d11);
+    const float depth = glsl_min(glsl_min(d00, d01), glsl_min(d10, d11));
     const float depthSphere = v.znear / (cz - r);
     return depthSphere > depth;
 }

@@ -282,7 +282,7 @@ bz_status submit_one(bz_ctx* ctx, Scheduler<HipDevice>* sv, const bz_request* r,
         !q->dev.plan.single_pass)
         return ctx->fail(BZ_E_INVALID, "depth extent gives an unsupported pyramid");
     q->dev.depth_bytes = Arena::round_up((uint64_t)r->depth_w * r->depth_h * 4, kAlign);
-    q->dev.texel_bytes = Arena::round_up(q->dev.plan.n_texels * 4, kAlign);
+    q->dev.texel_bytes = Arena::round_up((q->dev.plan.n_texels + 1) * 4, kAlign);  // + the Hi-Z pairs' spare texel
     q->dev.tail_bytes = Arena::round_up((uint64_t)q->dev.plan.tiles_x * q->dev.plan.tiles_y * 8, kAlign);
     q->work_bytes = q->dev.depth_bytes + q->dev.texel_bytes + q->dev.tail_bytes;
     q->out_bytes = Arena::round_up((uint64_t)std::max(q->n_objects, 1u) * 24, kAlign);
