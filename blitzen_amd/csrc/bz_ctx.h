@@ -102,9 +102,9 @@ struct StreamScratch {
     ClusterScratch xs{};              // cluster pass
     uint32_t* inst_hist = nullptr;    // LOD-instanced output: chunk x LOD histogram, then each chunk's first slots
     uint64_t inst_hist_cap = 0;
-    unsigned long long* inst_seg = nullptr;  // LOD-instanced output: the scan's published segment totals
+    unsigned long long* inst_seg = nullptr;  // LOD-instanced output: the scan's published segment totals (zero)
     uint64_t inst_seg_cap = 0;
-    uint32_t inst_epoch = 0;          // tag of the last LOD-instanced call on this stream
+    bool inst_seg_dirty = false;      // a launch sequence stopped between the scan and the scatter that clears them
     ~StreamScratch();
 };
 

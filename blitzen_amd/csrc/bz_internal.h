@@ -115,13 +115,13 @@ struct InstancedJob {
     uint32_t* cmds;               // [capacity] x 8
     uint32_t* count;
     uint32_t n_lods, per_lod, capacity, n_tiles, tile_size, tiles_per_chunk, n_chunks;
-    uint32_t epoch;               // this call's tag on the scan's published segment totals (never 0)
     uint32_t* err;                // [0] set when the scan gives up its bounded wait
     uint32_t spin_limit;
 };
 // The LOD-instanced output's device state: `hist`, the chunk x LOD histogram (n_chunks x n_lods words, every word
 // written by the decide kernel or the histogram kernel), and `seg`, inst_seg_words() 64-bit words in which the scan
-// publishes each segment of kInstSegChunks chunks' totals, tagged with the call's epoch (zero at allocation).
+// publishes each segment of kInstSegChunks chunks' totals with a presence flag; zero between calls (the scatter
+// launch clears them).
 constexpr uint32_t kInstSegChunks = 32;
 inline uint32_t inst_segments(uint32_t n_chunks) { return n_chunks ? (n_chunks + kInstSegChunks - 1) / kInstSegChunks : 1u; }
 inline uint64_t inst_seg_words(uint32_t n_chunks, uint32_t n_lods) { return (uint64_t)inst_segments(n_chunks) * n_lods; }

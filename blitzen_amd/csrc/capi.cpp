@@ -1042,7 +1042,7 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
             if (e == hipSuccess) sc->inst_hist_cap = need;
         }
         if (e == hipSuccess && sc->inst_seg_cap < need_seg) {
-            // the scan's published totals: zero = the tag of no call (epochs start at 1)
+            // the scan's published totals: zero = not yet published
             if (sc->inst_seg) (void)hipFree(sc->inst_seg);
             sc->inst_seg = nullptr;
             sc->inst_seg_cap = 0;
@@ -1052,8 +1052,11 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
         }
         if (e != hipSuccess) return ctx->hip_fail(e, "instancing scratch");
     }
-    if (++sc->inst_epoch == 0) ++sc->inst_epoch;
-    ij.epoch = sc->inst_epoch;
+    if (sc->inst_seg_dirty) {
+        hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(sc->inst_seg), 0u, sc->inst_seg_cap * 2, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "instancing scratch clear");
+        sc->inst_seg_dirty = false;
+    }
     ij.err = sc->err;
     ij.spin_limit = ctx->wait_limit;
     if (job.n) {
@@ -1062,8 +1065,10 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
                                           fused ? &ih : nullptr);
         if (e != hipSuccess) return ctx->hip_fail(e, "decide launch");
     }
+    sc->inst_seg_dirty = true;
     hipError_t e = launch_instanced(ij, sc->cs, sc->inst_hist, sc->inst_seg, fused, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "instanced launch");
+    sc->inst_seg_dirty = false;
     return BZ_OK;
 }
 

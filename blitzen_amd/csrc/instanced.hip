@@ -100,8 +100,8 @@ __global__ __launch_bounds__(256) void inst_hist_kernel(InstancedJob J, const ui
 
 // Scan of the chunk x LOD histogram over the chunks, per LOD, by workgroups of (64 LODs: one per lane) x (segment of
 // kSegChunks chunks: kSegChunks / 4 consecutive chunks per wave), so that a scan of ~1 M counters has hundreds of
-// workgroups. A workgroup publishes its segment's total per LOD in one 64-bit word tagged with the call's epoch
-// (value and tag in one atomic store: no fence, no reset), and sums the totals of the LOD group's earlier segments
+// workgroups. A workgroup publishes its segment's total per LOD in one 64-bit word with a presence flag (value and
+// flag in one atomic store: no fence), and sums the totals of the LOD group's earlier segments
 // (look-back over at most a few dozen words, all loaded at once; a word of an earlier segment whose workgroup has not
 // stored it yet is polled, with a bounded wait). Workgroups are dispatched in order of their linear index, segment
 // major, so a workgroup waits only on workgroups dispatched before it. Each chunk's count is replaced by its first
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void inst_scan_kernel(InstancedJob J, uint32_t
         run += w < wave ? x : 0u;
         total += x;
     }
-    const unsigned long long tag = (unsigned long long)J.epoch << 32;
+    constexpr unsigned long long tag = 1ull << 63;  // present; the scatter kernel clears the words for the next call
     if (wave == 0) {
         if (ok && s + 1u < n_seg)
             __hip_atomic_store(&seg[(size_t)s * J.n_lods + l], tag | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void inst_scan_kernel(InstancedJob J, uint32_t
 #pragma unroll
             for (uint32_t k = 0; k < 16; ++k) {
                 uint32_t spins = 0;
-                while ((w[k] & 0xffffffff00000000ull) != tag) {
+                while (!(w[k] & tag)) {
                     if (++spins > J.spin_limit) {
                         atomicExch(J.err, 1u);
                         break;
@@ -249,8 +249,15 @@ constexpr uint32_t kScatterWaves = 4;
 
 __global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(InstancedJob J, const uint32_t* counts,
                                                                           const uint2* staging,
-                                                                          const uint32_t* hist)
+                                                                          const uint32_t* hist,
+                                                                          unsigned long long* seg, uint64_t n_seg_words)
 {
+    {  // the scan's published words, cleared for the next call (the scan has finished: this is the next launch)
+        const uint64_t nthreads = (uint64_t)gridDim.x * gridDim.y * blockDim.x;
+        for (uint64_t i = ((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < n_seg_words;
+             i += nthreads)
+            __hip_atomic_store(&seg[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __shared__ uint32_t slot[kScatterWaves][kScatterSlice];  // per wave: LOD -> count (1), then next slot (2, 3)
     __shared__ uint32_t running[kScatterSlice];              // per LOD: the next slot after the windows so far
     __shared__ uint32_t cmd_wsum[kScatterWaves];
@@ -370,7 +377,8 @@ hipError_t launch_instanced(const InstancedJob& J, const CompactScratch& s, uint
     }
     const uint32_t scatter_slices = (J.n_lods + kScatterSlice - 1) / kScatterSlice;
     hipLaunchKernelGGL(inst_scatter_kernel, dim3(J.n_chunks + 1u, scatter_slices), dim3(kScatterWaves * 64), 0, st, J,
-                       (const uint32_t*)s.counts, (const uint2*)s.staging, (const uint32_t*)hist);
+                       (const uint32_t*)s.counts, (const uint2*)s.staging, (const uint32_t*)hist, seg,
+                       (uint64_t)(inst_segments(J.n_chunks) - 1u) * J.n_lods);
     return hipGetLastError();
 }
 
