@@ -400,6 +400,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-clusters", action="store_true")
     ap.add_argument("--no-scaling", action="store_true", help="skip the PLAIN pass at 4, 8 and 16 M objects")
+    ap.add_argument("--no-graph-replay", action="store_true", help="skip timing the frames replayed from hipGraphs")
     ap.add_argument("--no-next", action="store_true", help="skip the PLAIN / ONPC / instanced / transform-update leg")
     ap.add_argument("--served-seconds", type=float, default=0.5)
     ap.add_argument("--no-frame-cache", action="store_true",
@@ -453,19 +454,41 @@ def main():
     ce, cl = _Cnt(cnt_e_ptr), _Cnt(cnt_l_ptr)
     fc_flag = 0 if args.no_frame_cache else A.CULL_FRAME_CACHE
 
+    # Each copy's two bz_cull argument structs are built once and the three calls per frame go straight to the
+    # library, as a native renderer would make them: the per-call Python work of the convenience wrappers (building
+    # ctypes structs) took enough host time that on some hosts the GPU waited between launches.
+    import ctypes as C
+
+    L = bz.lib()
+    vptr = C.pointer(v)
+    frame_args = []
+    for h, vin, vout, dbuf, pyr in copies:
+        ae = A.CullArgs(scene=h, pyramid=0, pass_=A.PASS_EARLY, flags=fc_flag, view=vptr, cmds_dev=cmds_e.ptr,
+                        count_dev=cnt_e_ptr, capacity=N_OBJECTS, object_begin=0, object_count=0,
+                        visibility_in_dev=vin.ptr, visibility_out_dev=None)
+        al = A.CullArgs(scene=h, pyramid=pyr, pass_=A.PASS_LATE, flags=fc_flag, view=vptr, cmds_dev=cmds_l.ptr,
+                        count_dev=cnt_l_ptr, capacity=N_OBJECTS, object_begin=0, object_count=0,
+                        visibility_in_dev=vin.ptr, visibility_out_dev=vout.ptr)
+        frame_args.append((ae, al, C.byref(ae), C.byref(al)))
+
     def step(k, ev=None):
         h, vin, vout, dbuf, pyr = copies[k % len(copies)]
+        _, _, ae, al = frame_args[k % len(copies)]
         if ev:
             ev[0].record(stream)
-        ctx.pyramid_build(pyr, dbuf.ptr, DEPTH_W, DEPTH_H, stream)
+        s = L.bz_pyramid_build(ctx.handle, pyr, dbuf.ptr, DEPTH_W, DEPTH_H, stream)
+        if s:
+            ctx.check(s, "bz_pyramid_build")
         if ev:
             ev[1].record(stream)
-        ctx.cull(h, v, A.PASS_EARLY, cmds_e, ce, capacity=N_OBJECTS, flags=fc_flag, vis_in=vin.ptr,
-                 stream=stream)
+        s = L.bz_cull(ctx.handle, ae, stream)
+        if s:
+            ctx.check(s, "bz_cull (EARLY)")
         if ev:
             ev[2].record(stream)
-        ctx.cull(h, v, A.PASS_LATE, cmds_l, cl, capacity=N_OBJECTS, pyramid=pyr, flags=fc_flag,
-                 vis_in=vin.ptr, vis_out=vout.ptr, stream=stream)
+        s = L.bz_cull(ctx.handle, al, stream)
+        if s:
+            ctx.check(s, "bz_cull (LATE)")
         if ev:
             ev[3].record(stream)
 
@@ -506,6 +529,34 @@ def main():
     t_early = [e[1].elapsed_ms(e[2]) for e in evs]
     t_late = [e[2].elapsed_ms(e[3]) for e in evs]
     t_frame = [e[0].elapsed_ms(e[3]) for e in evs]
+
+    # The same frames again, each copy's frame (pyramid build + EARLY + LATE) captured once in a hipGraph and
+    # replayed with one launch per frame (reported beside the headline: the graph's kernel nodes start later than
+    # back-to-back launches on the stream, so the replayed frame is slower here).
+    graph_replay = None
+    if not args.no_graph_replay:
+        from blitzen_amd.hip import StreamGraph
+
+        graphs = [StreamGraph(stream, lambda k=k: step(k)) for k in range(len(copies))]
+        for k in range(args.warmup):
+            graphs[k % len(graphs)].launch(stream)
+        ctx.sync(stream)
+        g_start, g_end = Event(), Event()
+        t_g = time.perf_counter()
+        g_start.record(stream)
+        for k in range(args.steps):
+            graphs[k % len(graphs)].launch(stream)
+        g_end.record(stream)
+        ctx.sync(stream)
+        wall_g = time.perf_counter() - t_g
+        ctx.check_kernels(stream)
+        counts_g = cnt.download(np.uint32, 2, stream)
+        if (int(counts_g[0]), int(counts_g[1])) != (n_early, n_late):
+            raise RuntimeError(f"replayed frames gave counts {list(counts_g)}, eager frames {n_early}, {n_late}")
+        del graphs
+        graph_replay = {"ms_per_step": round(wall_g / args.steps * 1e3, 4),
+                        "frame_timed_region_us": round(g_start.elapsed_ms(g_end) / args.steps * 1e3, 2),
+                        "note": "each copy's frame captured once in a hipGraph, one launch per frame"}
 
     ms_per_step = wall / args.steps * 1e3
     value = N_OBJECTS / (wall / args.steps)
@@ -563,6 +614,8 @@ def main():
                      "late": round(late_bytes / (late_ms * 1e-3) / 1e9, 1)},
         "commands": {"early": n_early, "late": n_late},
         "roofline": roofline,
+        "frames": "three library calls per frame (pyramid build, EARLY, LATE), argument structs built once",
+        "graph_replay": graph_replay,
     }
     if not args.no_batched:
         log("bench: batched views ...")
