@@ -94,6 +94,9 @@ def bench_batched(ctx, stream, steps: int, warmup: int):
     bv = W.BatchedViews.build(ctx, plan, stream=stream)
     log(f"bench: batched workload ({len(plan)} views, {bv.n_objects} object-view pairs) built in "
         f"{time.perf_counter() - t0:.1f}s")
+    # the autotuner keeps its choice in the context: the single-view tuning it picks for these views (used by the
+    # per-view baseline) is put back afterwards, so the legs that follow run with the built-in one as the frame did
+    single_view = ctx.get_tuning(0)
     tuned = ctx.autotune(1, bv.jobs, stream=stream)
     tuned0 = ctx.autotune(0, bv.jobs, stream=stream)
     graph = ctx.batch_graph_create(bv.pyr_jobs, bv.jobs)
@@ -120,6 +123,7 @@ def bench_batched(ctx, stream, steps: int, warmup: int):
     if not np.array_equal(c_graph, c_plain):
         raise RuntimeError("batched and per-view draw counts differ")
     ctx.graph_destroy(graph)
+    ctx.set_tuning(0, single_view.block, single_view.items, single_view.grid_per_cu)
     n_views = len(plan)
     cmds = int(c_graph.sum())
     alg = W.algorithmic_bytes_batched(plan, cmds)
@@ -240,6 +244,18 @@ def bench_clusters(ctx, stream, steps: int, warmup: int):
     ctx.check_kernels(stream)
     records, draws = cf.result_counts(stream)
     ms = statistics.median([a.elapsed_ms(b) for a, b in evs])
+    # the cluster cull alone (no transform upload): one event pair around `steps` calls
+    c0, c1 = Event(), Event()
+    c0.record(stream)
+    for _ in range(steps):
+        cf.cull(stream)
+    c1.record(stream)
+    ctx.sync(stream)
+    ctx.check_kernels(stream)
+    cull_ms = c0.elapsed_ms(c1) / steps
+    # algorithmic bytes of the cull: object pass 40 B per object (RenderObject 8 + transform 32), per record 12 B
+    # written and 12 B read back + 20 B of cluster data (sphere, cone), 24 B per draw, two counts
+    cull_bytes = cf.n_objects * 40 + records * (12 + 12 + 20) + draws * 24 + 8
     out = {"workload": "1,048,576 objects over 128 surfaces x 8 LODs with 16-128 clusters of 64 triangles (LOD k "
                        "owns max(1, c >> 2k)); per frame 1% seeded transform updates (10,485 x 32 B uploaded) + object "
                        "pass (frustum + LOD, as PreClusterDrawCull) + expansion + cluster frustum/Hi-Z/cone tests "
@@ -247,7 +263,10 @@ def bench_clusters(ctx, stream, steps: int, warmup: int):
            "frame_ms_p50": round(ms, 4), "frame_wall_ms": round(wall * 1e3, 4),
            "objects_per_s": round(cf.n_objects / (ms * 1e-3), 1),
            "cluster_records": records, "cluster_draws": draws,
-           "clusters_per_s": round(records / (ms * 1e-3), 1)}
+           "clusters_per_s": round(records / (ms * 1e-3), 1),
+           "cull_us": round(cull_ms * 1e3, 2), "cull_algorithmic_bytes": cull_bytes,
+           "cull_achieved_gbs": round(cull_bytes / (cull_ms * 1e-3) / 1e9, 1),
+           "cull_records_per_s": round(records / (cull_ms * 1e-3), 1)}
     cf.free()
     return out
 

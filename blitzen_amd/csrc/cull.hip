@@ -1115,27 +1115,37 @@ hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev
 constexpr uint32_t kScanBlock = 1024;
 constexpr uint32_t kScanPer = 8;  // consecutive counts per thread and round
 
-// One workgroup walks the counts in rounds of kScanBlock * kScanPer: per thread an exclusive scan of its 8 counts,
-// a wave scan of the thread totals, the waves' totals through LDS, and the carry of the earlier rounds. Linear in n;
-// a launch of its own because it needs every count of the job.
+// One workgroup walks the counts in rounds of kScanBlock * kScanPer. Each round is loaded and stored coalesced
+// (thread t takes words t, t + kScanBlock, ...) through an LDS tile, in which thread t then owns kScanPer consecutive
+// counts: an exclusive scan of its 8 counts, a wave scan of the thread totals and a block scan of the wave totals give
+// each count its prefix (plus the rounds before); the last word is the total. The LDS tile has one spare word per 32,
+// so the 8-word strided accesses spread over the banks.
+__device__ __forceinline__ uint32_t scan_lds_index(uint32_t i) { return i + (i >> 5); }
+
 __global__ __launch_bounds__(kScanBlock) void scan_counts_kernel(const uint32_t* __restrict__ counts, uint32_t n,
                                                                  uint32_t* __restrict__ prefix)
 {
-    constexpr uint32_t NW = kScanBlock / 64;
+    constexpr uint32_t NW = kScanBlock / 64, R = kScanBlock * kScanPer;
     __shared__ uint32_t wsum[NW];
+    __shared__ uint32_t tile[R + R / 32];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint32_t carry = 0;
-    for (uint32_t r0 = 0; r0 < n; r0 += kScanBlock * kScanPer) {
-        const uint32_t i0 = r0 + tid * kScanPer;
-        uint32_t v[kScanPer];
-#pragma unroll
-        for (uint32_t k = 0; k < kScanPer; ++k) v[k] = i0 + k < n ? counts[i0 + k] : 0u;
-        uint32_t own = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += R) {
+        uint32_t c[kScanPer];
 #pragma unroll
         for (uint32_t k = 0; k < kScanPer; ++k) {
-            const uint32_t c = v[k];
+            const uint32_t i = r0 + k * kScanBlock + tid;
+            c[k] = i < n ? counts[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) tile[scan_lds_index(k * kScanBlock + tid)] = c[k];
+        __syncthreads();
+        uint32_t v[kScanPer], own = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t x = tile[scan_lds_index(tid * kScanPer + k)];
             v[k] = own;
-            own += c;
+            own += x;
         }
         const uint32_t incl = wave_incl_scan(own, lane);
         if (lane == 63) wsum[wave] = incl;
@@ -1148,10 +1158,15 @@ __global__ __launch_bounds__(kScanBlock) void scan_counts_kernel(const uint32_t*
         }
         const uint32_t base = carry + before + (incl - own);
 #pragma unroll
-        for (uint32_t k = 0; k < kScanPer; ++k)
-            if (i0 + k < n) prefix[i0 + k] = base + v[k];
+        for (uint32_t k = 0; k < kScanPer; ++k) tile[scan_lds_index(tid * kScanPer + k)] = base + v[k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t i = r0 + k * kScanBlock + tid;
+            if (i < n) prefix[i] = tile[scan_lds_index(k * kScanBlock + tid)];
+        }
         carry += total;
-        __syncthreads();  // wsum is rewritten by the next round
+        __syncthreads();  // wsum and the tile are rewritten by the next round
     }
     if (threadIdx.x == 0) prefix[n] = carry;
 }

@@ -231,6 +231,10 @@ class ClusterFrames:
 
     def frame(self, ids, new, stream=None):
         self.ctx.update_transforms(self.scene, ids, new, stream)
+        self.cull(stream)
+
+    def cull(self, stream=None):
+        """The cluster cull alone (object pass, expansion, cluster tests, emit)."""
         cnt = self.counts
         self.ctx.cull_clusters(self.scene, self.view, A.CLUSTER_FRUSTUM | A.CLUSTER_HIZ | A.CLUSTER_CONE, self.disp,
                                _Ptr(cnt.ptr), self.cmds, _Ptr(cnt.ptr + 4), pyramid=self.pyramid, stream=stream)

@@ -29,6 +29,16 @@ def test_traffic_summary_is_committed_and_consistent():
     assert abs(fetch + write - t["bytes_per_launch"]) <= 0.001 * t["bytes_per_launch"]
     # the command that made it runs committed files only
     assert "bench.py" in t["command"] and "_work" not in t["command"]
+    # it names the kernel sources it measured (bench.py reports it as stale when they differ from the library's)
+    src = t.get("kernel_sources", "")
+    assert len(src) == 16 and all(c in "0123456789abcdef" for c in src)
+
+
+def test_profiles_name_their_kernel_sources_and_commands():
+    for p in sorted((ROOT / "profiles").glob("*.md")):
+        text = p.read_text()
+        assert "Kernel sources: `" in text, p.name
+        assert "rocprofv3 " in text and "_work" not in text, p.name
 
 
 def test_percentile_picks_sample_values():
