@@ -533,6 +533,7 @@ def main():
     for k in range(args.steps):
         step(k)
     e_end.record(stream)
+    enqueue = time.perf_counter() - t_start  # host time to issue the K frames (close to `wall`: host-bound)
     ctx.sync(stream)
     wall = time.perf_counter() - t_start
     ctx.check_kernels(stream)
@@ -634,6 +635,7 @@ def main():
         "commands": {"early": n_early, "late": n_late},
         "roofline": roofline,
         "frames": "three library calls per frame (pyramid build, EARLY, LATE), argument structs built once",
+        "host_enqueue_us_per_step": round(enqueue / args.steps * 1e6, 2),
         "graph_replay": graph_replay,
     }
     if not args.no_batched:
