@@ -378,8 +378,9 @@ BZ_API bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_
  * (the pyramid that was being built holds wrong values until rebuilt). */
 BZ_API bz_status bz_check(bz_ctx* ctx, void* stream);
 /* The number of short sleeps (s_sleep 1, about 64 clocks each) a bounded wait inside a kernel may take before it
- * gives up; 0 gives up at the first word not yet handed over. Applies to pyramid builds launched afterwards through
- * this context (bz_pyramid_build, bz_pyramid_build_batch, bz_batch_graph_create). Default BZ_DEFAULT_WAIT_LIMIT.
+ * gives up; 0 gives up at the first word not yet handed over. Applies to launches made afterwards through this
+ * context: pyramid builds (bz_pyramid_build, bz_pyramid_build_batch, bz_batch_graph_create) and the scan of
+ * bz_cull_instanced. Default BZ_DEFAULT_WAIT_LIMIT.
  * A small limit forces the give-up path for tests of the error handling; with the default the waits end in a few
  * microseconds. */
 #define BZ_DEFAULT_WAIT_LIMIT 1048576u
