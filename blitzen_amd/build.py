@@ -1,8 +1,9 @@
 """Builds libbzcull.so (HIP, gfx950) in-tree.
 
-Every HIP source is compiled with -ffp-contract=off (no FMA contraction, so device fp32 results equal a CPU build of
-the same expressions), HIP's default correctly rounded fp32 division and square root, IEEE denormals, and packed
-FP32 instructions disabled. Objects are rebuilt every call; the link is one hipcc invocation.
+Every HIP source is compiled with -ffp-contract=off (the compiler fuses no multiply-add; the cull arithmetic fuses
+only where it calls fma explicitly, as the oracle does, so device fp32 results equal a CPU build of the same
+expressions), HIP's default correctly rounded fp32 division and square root, IEEE denormals, and packed FP32
+instructions disabled. Objects are rebuilt every call; the link is one hipcc invocation.
 """
 from __future__ import annotations
 

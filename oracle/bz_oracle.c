@@ -4,7 +4,11 @@
  * This file is the checker, never the product: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it. It restates, in plain fp32 C compiled with -ffp-contract=off and no
  * fast-math, the arithmetic of Blitzen's cull shaders (GLSL, which cannot be compiled or run here) and
- * of the host camera code (C++):
+ * of the host camera code (C++). A product is fused into the add or subtract that follows it (fmaf, one
+ * rounding) exactly where an expression is written with fmaf: in orc_rotate_quat, orc_sphere_to_view,
+ * orc_frustum and orc_project_sphere (GLSL lets a compiler
+ * contract a*b+c in shaders that declare nothing `precise`, as the reference's do; the GPU kernels fuse at the
+ * same points):
  *
  *   orc_rotate_quat       src/Renderer/VulkanShaderHeaders/ShaderBuffers.glsl:197-200
  *   orc_sphere_to_view    src/Renderer/VulkanShaderHeaders/CullingShaderData.glsl:44-46
@@ -30,7 +34,7 @@
  * Points the reference leaves to fixed-function hardware are settled by explicit rules, shared with the
  * HIP kernels:
  *   - LOD level  L = clamp(floor(log2(m)), 0, mips-1) by exponent extraction (exact), m = max(w, h); m < 1 -> 0.
- *   - footprint  tx = u*W - 0.5, i0 = floor(tx), i1 = i0 + 1, both clamped to [0, W-1]; texel i1 counts only
+ *   - footprint  tx = u*W - 0.5 (two roundings), i0 = floor(tx), i1 = i0 + 1, both clamped to [0, W-1]; texel i1 counts only
  *                when its bilinear weight frac(tx) is non-zero (the MIN reduction ignores zero-weight texels).
  *   - output order ascending objectId (the reference's atomicAdd order is not deterministic).
  *
@@ -94,27 +98,28 @@ static inline float glsl_min(float x, float y) { return (y < x) ? y : x; }
  * cross(a, b) = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x) (blitML.h:219-221). */
 ORC_EXPORT void orc_rotate_quat(const float v[3], const float q[4], float out[3])
 {
-    float t0 = (q[1] * v[2] - q[2] * v[1]) + q[3] * v[0];
-    float t1 = (q[2] * v[0] - q[0] * v[2]) + q[3] * v[1];
-    float t2 = (q[0] * v[1] - q[1] * v[0]) + q[3] * v[2];
-    float c0 = q[1] * t2 - q[2] * t1;
-    float c1 = q[2] * t0 - q[0] * t2;
-    float c2 = q[0] * t1 - q[1] * t0;
+    float t0 = fmaf(q[3], v[0], fmaf(q[1], v[2], -(q[2] * v[1])));
+    float t1 = fmaf(q[3], v[1], fmaf(q[2], v[0], -(q[0] * v[2])));
+    float t2 = fmaf(q[3], v[2], fmaf(q[0], v[1], -(q[1] * v[0])));
+    float c0 = fmaf(q[1], t2, -(q[2] * t1));
+    float c1 = fmaf(q[2], t0, -(q[0] * t2));
+    float c2 = fmaf(q[0], t1, -(q[1] * t0));
     out[0] = v[0] + 2.0f * c0;
     out[1] = v[1] + 2.0f * c1;
     out[2] = v[2] + 2.0f * c2;
 }
 
 /* CullingShaderData.glsl:44-46: center = RotateQuat(c, q) * scale + pos; center = (view * vec4(center, 1)).xyz;
- * radius = r * scale. mat*vec4 sums in the order of blitMLTypes.h:184-193 (m0*x + y*m4 + z*m8 + w*m12). */
+ * radius = r * scale. mat*vec4 sums in the order of blitMLTypes.h:184-193 (m0*x + y*m4 + z*m8 + w*m12), unfused;
+ * the scale-and-translate is fused. */
 ORC_EXPORT void orc_sphere_to_view(const float c[3], float r, const float pos[3], float scale, const float q[4],
                                    const float* view, float out_c[3], float* out_r)
 {
     float rq[3];
     orc_rotate_quat(c, q, rq);
-    float w0 = rq[0] * scale + pos[0];
-    float w1 = rq[1] * scale + pos[1];
-    float w2 = rq[2] * scale + pos[2];
+    float w0 = fmaf(rq[0], scale, pos[0]);
+    float w1 = fmaf(rq[1], scale, pos[1]);
+    float w2 = fmaf(rq[2], scale, pos[2]);
     for (int k = 0; k < 3; ++k)
         out_c[k] = ((view[k] * w0 + w1 * view[4 + k]) + w2 * view[8 + k]) + 1.0f * view[12 + k];
     *out_r = r * scale;
@@ -125,8 +130,8 @@ ORC_EXPORT int orc_frustum(const float c[3], float r, float fRight, float fLeft,
                            float zNear, float zFar)
 {
     int vis = 1;
-    vis = vis && (c[2] * fLeft - fabsf(c[0]) * fRight > -r);
-    vis = vis && (c[2] * fBottom - fabsf(c[1]) * fTop > -r);
+    vis = vis && (fmaf(c[2], fLeft, -(fabsf(c[0]) * fRight)) > -r);
+    vis = vis && (fmaf(c[2], fBottom, -(fabsf(c[1]) * fTop)) > -r);
     vis = vis && (c[2] + r > zNear) && (c[2] - r < zFar);
     return vis;
 }
@@ -149,13 +154,13 @@ ORC_EXPORT int orc_project_sphere(const float c[3], float r, float znear, float 
 {
     if (c[2] < r + znear) return 0;
     float crx = c[0] * r, cry = c[1] * r, crz = c[2] * r;
-    float czr2 = c[2] * c[2] - r * r;
-    float vx = sqrtf(c[0] * c[0] + czr2);
-    float minx = (vx * c[0] - crz) / (vx * c[2] + crx);
-    float maxx = (vx * c[0] + crz) / (vx * c[2] - crx);
-    float vy = sqrtf(c[1] * c[1] + czr2);
-    float miny = (vy * c[1] - crz) / (vy * c[2] + cry);
-    float maxy = (vy * c[1] + crz) / (vy * c[2] - cry);
+    float czr2 = fmaf(c[2], c[2], -(r * r));
+    float vx = sqrtf(fmaf(c[0], c[0], czr2));
+    float minx = fmaf(vx, c[0], -crz) / fmaf(vx, c[2], crx);
+    float maxx = fmaf(vx, c[0], crz) / fmaf(vx, c[2], -crx);
+    float vy = sqrtf(fmaf(c[1], c[1], czr2));
+    float miny = fmaf(vy, c[1], -crz) / fmaf(vy, c[2], cry);
+    float maxy = fmaf(vy, c[1], crz) / fmaf(vy, c[2], -cry);
     /* vec4(minx*P00, miny*P11, maxx*P00, maxy*P11).xwzy * vec4(0.5, -0.5, 0.5, -0.5) + vec4(0.5) */
     aabb[0] = (minx * P00) * 0.5f + 0.5f;
     aabb[1] = (maxy * P11) * -0.5f + 0.5f;
