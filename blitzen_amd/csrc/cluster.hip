@@ -122,6 +122,9 @@ __global__ __launch_bounds__(kXBlock) void cluster_expand_kernel(ClusterJob J, c
     __shared__ ExpandSmem sm;
     const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t ne = counts[t];
+    // the tile's first kXBlock staging entries, loaded with its count (before it is known; every tile owns
+    // tile_size >= 2 * kXBlock slots), so that they arrive in the same round trip as the counts and the prefix
+    const uint2 first_en = staging[(size_t)t * J.tile_size + tid];
     if (tid < (uint32_t)kMaxMips) {  // pyramid level offsets from the extent
         uint32_t off = 0;
         for (uint32_t L = 0; L < tid; ++L) off += level_dim(J.pyr.base_w, L) * level_dim(J.pyr.base_h, L);
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(kXBlock) void cluster_expand_kernel(ClusterJob J, c
         const uint32_t e = e0 + tid;
         uint32_t cnt = 0;
         if (e < ne) {
-            const uint2 en = staging[(size_t)t * J.tile_size + e];
+            const uint2 en = e0 == 0 ? first_en : staging[(size_t)t * J.tile_size + e];
             const uint2 lc = J.scene.lod_cluster[en.y];
             sm.obj[e] = en.x;
             sm.lod[e] = en.y;
