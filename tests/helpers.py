@@ -46,3 +46,32 @@ def c1_depth(seed: int = SEED_C1, w=1920, h=1080, occluders=512, p11=None, znear
 
 def cmds_u32(structured: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(structured).view(np.uint32).reshape(-1, 6)
+
+
+def boundary_scene(n: int, v, seed: int):
+    """n objects (16 surfaces x 4 LODs) whose view-space bounding spheres touch a side plane of `v`'s frustum in
+    exact arithmetic: even objects a left/right plane, odd ones a top/bottom plane, depth U[20, 300], each placed
+    from float64 arithmetic and rounded to float32 positions, so that whether an object is inside depends on the
+    rounding of every step of the transform and the plane test."""
+    sc = bz.synth_scene(seed, n, 16, 4)
+    objs, xfs, surfs, lods = sc.objects, sc.transforms.copy(), sc.surfaces, sc.lods
+    rng = np.random.default_rng(seed)
+    s = surfs[objs["surfaceId"]]
+    q = xfs["orientation"].astype(np.float64)
+    sig = xfs["scale"].astype(np.float64)
+    r = (s["radius"].astype(np.float32) * xfs["scale"].astype(np.float32)).astype(np.float64)
+    c0 = s["center"].astype(np.float64)
+    qv, qw = q[:, :3], q[:, 3:4]
+    o = (c0 + 2.0 * np.cross(qv, np.cross(qv, c0) + qw * c0)) * sig[:, None]  # rotated, scaled centre
+    m = np.array(v.view, np.float64).reshape(4, 4)  # m[col][row]
+    V3, t = m[:3, :3].T, m[3, :3]
+    fR, fL, fT, fB = (float(x) for x in (v.frustumRight, v.frustumLeft, v.frustumTop, v.frustumBottom))
+    cz = rng.uniform(20.0, 300.0, n)
+    sgn = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    inner = rng.uniform(-0.3, 0.3, n)
+    xplane = np.arange(n) % 2 == 0
+    cx = np.where(xplane, sgn * (cz * fL + r) / fR, inner * cz * fL / fR)
+    cy = np.where(xplane, inner * cz * fB / fT, sgn * (cz * fB + r) / fT)
+    world = (np.stack([cx, cy, cz], 1) - t) @ V3  # V3^T (c - t): V3 is a rotation
+    xfs["pos"] = (world - o).astype(np.float32)
+    return objs, xfs, surfs, lods

@@ -324,3 +324,42 @@ def test_occlusion_samples_box_centre():
             assert bool(got) == (cu < 0.5), (axis, x, cu)
             checked += 1
         assert checked > 390
+
+
+# ------------------------------------------------------------------------------------------------- rounding at planes
+def unfused_frustum(objs, xfs, surfs, v):
+    """The transform and frustum test in numpy float32 with every product and sum rounded on its own (no fused
+    multiply-add anywhere): the arithmetic the cull would have without the fused points."""
+    f = np.float32
+    s = surfs[objs["surfaceId"]]
+    vx, vy, vz = (s["center"][:, k].astype(f) for k in range(3))
+    x, y, z, w = (xfs["orientation"][:, k].astype(f) for k in range(4))
+    tx = (y * vz - z * vy) + w * vx
+    ty = (z * vx - x * vz) + w * vy
+    tz = (x * vy - y * vx) + w * vz
+    rx, ry, rz = vx + f(2) * (y * tz - z * ty), vy + f(2) * (z * tx - x * tz), vz + f(2) * (x * ty - y * tx)
+    sig = xfs["scale"].astype(f)
+    wx, wy, wz = (rr * sig + xfs["pos"][:, k].astype(f) for k, rr in enumerate((rx, ry, rz)))
+    m = np.array(v.view, f)
+    cx, cy, cz = (((m[k] * wx + wy * m[4 + k]) + wz * m[8 + k]) + f(1) * m[12 + k] for k in range(3))
+    r = s["radius"].astype(f) * sig
+    fR, fL, fT, fB = (f(t) for t in (v.frustumRight, v.frustumLeft, v.frustumTop, v.frustumBottom))
+    return ((cz * fL - np.abs(cx) * fR > -r) & (cz * fB - np.abs(cy) * fT > -r) & (cz + r > f(v.zNear))
+            & (cz - r < f(v.zFar)))
+
+
+def test_boundary_scene_separates_the_fused_points():
+    """The objects of tests.helpers.boundary_scene sit on the frustum's side planes, so their decisions depend on
+    where the arithmetic rounds: about half are inside, and the oracle (fused multiply-adds at its fixed points) and
+    the same arithmetic without fusion decide a few percent of them differently. A kernel that fused at other points,
+    or not at all, therefore fails the GPU parity test on this scene (tests/test_gpu_boundary.py)."""
+    for yaw, pitch, seed in ((0.3, 0.1, 7), (-1.1, -0.4, 8)):
+        v = H.view(yaw=yaw, pitch=pitch)
+        objs, xfs, surfs, lods = H.boundary_scene(20000, v, seed)
+        cmds, cnt = orc.cull((objs, xfs, surfs, lods), H.view_bytes(v), 2, capacity=20000)  # PLAIN
+        fused = np.zeros(20000, bool)
+        fused[H.cmds_u32(cmds)[:cnt, 0]] = True
+        assert 0.25 * 20000 < cnt < 0.85 * 20000, cnt
+        plain = unfused_frustum(objs, xfs, surfs, v)
+        diff = int(np.count_nonzero(plain != fused))
+        assert 200 < diff < 4000, diff
