@@ -75,3 +75,44 @@ def boundary_scene(n: int, v, seed: int):
     world = (np.stack([cx, cy, cz], 1) - t) @ V3  # V3^T (c - t): V3 is a rotation
     xfs["pos"] = (world - o).astype(np.float32)
     return objs, xfs, surfs, lods
+
+
+def hiz_level_scene(n: int, v, seed: int):
+    """n objects whose view-space spheres sit on the view axis (x = y = 0 in exact arithmetic, depth U[20, 200]) with
+    radii such that the projected box's larger side, in pyramid texels of `v` (pyramidWidth x pyramidHeight set), is
+    within two rounding quanta below 2^k for odd k in {1, 3, 5, 7}: the Hi-Z level (floor of its log2) is k or k - 1
+    depending on how each step of the transform and the projection rounds. Scales are chosen per object to give the
+    radius."""
+    sc = bz.synth_scene(seed, n, 16, 4)
+    objs, xfs, surfs, lods = sc.objects, sc.transforms.copy(), sc.surfaces, sc.lods
+    rng = np.random.default_rng(seed)
+    s = surfs[objs["surfaceId"]]
+    big = max(float(v.proj0) * float(v.pyramidWidth), float(v.proj5) * float(v.pyramidHeight))
+    k = rng.choice(np.array([1, 3, 5, 7]), n)
+    cz = rng.uniform(20.0, 200.0, n)
+    # the side's fp32 value is quantised by the subtraction of two UV coordinates near 0.5 (2^-24 in UV, 2^-14 of a
+    # texel at 1024 texels): aim up to two quanta below 2^k so that it rounds to either side
+    side = np.exp2(k) - rng.uniform(0.0, 2.0, n) * 2.0 ** -14
+    ratio = side / big  # r / sqrt(cz^2 - r^2) for a side of `side` texels
+    r_target = cz * ratio / np.sqrt(1.0 + ratio * ratio)
+    xfs["scale"] = (r_target / s["radius"].astype(np.float64)).astype(np.float32)
+    q = xfs["orientation"].astype(np.float64)
+    sig = xfs["scale"].astype(np.float64)
+    c0 = s["center"].astype(np.float64)
+    qv, qw = q[:, :3], q[:, 3:4]
+    o = (c0 + 2.0 * np.cross(qv, np.cross(qv, c0) + qw * c0)) * sig[:, None]
+    m = np.array(v.view, np.float64).reshape(4, 4)
+    V3, t = m[:3, :3].T, m[3, :3]
+    world = (np.stack([np.zeros(n), np.zeros(n), cz], 1) - t) @ V3
+    xfs["pos"] = (world - o).astype(np.float32)
+    return objs, xfs, surfs, lods
+
+
+def parity_pyramid(base_w: int, base_h: int, mips: int) -> np.ndarray:
+    """Pyramid texels (levels back to back) holding 0.0 on even levels and 1.0 on odd ones: a sphere passes the Hi-Z
+    test exactly when its level is even."""
+    out = []
+    for L in range(mips):
+        w, h = max(1, base_w >> L), max(1, base_h >> L)
+        out.append(np.full(w * h, float(L & 1), np.float32))
+    return np.concatenate(out + [np.zeros(1, np.float32)])

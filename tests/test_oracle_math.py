@@ -348,6 +348,55 @@ def unfused_frustum(objs, xfs, surfs, v):
             & (cz - r < f(v.zFar)))
 
 
+def unfused_hiz_level(objs, xfs, surfs, v, mips=11):
+    """The Hi-Z level of each object's projected box in numpy float32 without fused multiply-adds (transform,
+    projectSphere, box side in texels, floor(log2) by frexp)."""
+    f = np.float32
+    s = surfs[objs["surfaceId"]]
+    vx, vy, vz = (s["center"][:, k].astype(f) for k in range(3))
+    x, y, z, w = (xfs["orientation"][:, k].astype(f) for k in range(4))
+    tx = (y * vz - z * vy) + w * vx
+    ty = (z * vx - x * vz) + w * vy
+    tz = (x * vy - y * vx) + w * vz
+    rx, ry, rz = vx + f(2) * (y * tz - z * ty), vy + f(2) * (z * tx - x * tz), vz + f(2) * (x * ty - y * tx)
+    sig = xfs["scale"].astype(f)
+    wx, wy, wz = (rr * sig + xfs["pos"][:, k].astype(f) for k, rr in enumerate((rx, ry, rz)))
+    m = np.array(v.view, f)
+    cx, cy, cz = (((m[k] * wx + wy * m[4 + k]) + wz * m[8 + k]) + f(1) * m[12 + k] for k in range(3))
+    r = s["radius"].astype(f) * sig
+    crx, cry, crz = cx * r, cy * r, cz * r
+    czr2 = cz * cz - r * r
+    sx, sy = np.sqrt(cx * cx + czr2), np.sqrt(cy * cy + czr2)
+    minx, maxx = (sx * cx - crz) / (sx * cz + crx), (sx * cx + crz) / (sx * cz - crx)
+    miny, maxy = (sy * cy - crz) / (sy * cz + cry), (sy * cy + crz) / (sy * cz - cry)
+    p00, p11 = f(v.proj0), f(v.proj5)
+    a0, a2 = (minx * p00) * f(0.5) + f(0.5), (maxx * p00) * f(0.5) + f(0.5)
+    a1, a3 = (maxy * p11) * f(-0.5) + f(0.5), (miny * p11) * f(-0.5) + f(0.5)
+    side = np.maximum((a2 - a0) * f(v.pyramidWidth), (a3 - a1) * f(v.pyramidHeight))
+    e = np.frexp(side.astype(np.float64))[1] - 1
+    return np.where(side >= 1, np.minimum(e, mips - 1), 0)
+
+
+def test_hiz_level_scene_separates_the_fused_points():
+    """tests.helpers.hiz_level_scene puts each projected box's side just below a power of two in texels. Against the
+    pyramid of tests.helpers.parity_pyramid (0.0 on even levels, 1.0 on odd ones) an object passes the Hi-Z test
+    exactly when its level is even, so the TEMPORAL pass reveals every level choice: the oracle's (fused points) and
+    the unfused arithmetic's differ on about 1% of the objects, and both levels occur often. The GPU parity tests on
+    this scene (tests/test_gpu_boundary.py) therefore fail for a projection that fuses at other points."""
+    tex = H.parity_pyramid(1024, 512, 11)
+    opyr = orc.pyramid(tex, 1024, 512, 11)
+    for yaw, pitch, seed in ((0.3, 0.1, 7), (-1.1, -0.4, 8)):
+        v = H.view(yaw=yaw, pitch=pitch, pyr=(1024, 512))
+        objs, xfs, surfs, lods = H.hiz_level_scene(20000, v, seed)
+        cmds, cnt = orc.cull((objs, xfs, surfs, lods), H.view_bytes(v), 3, opyr, capacity=20000)  # TEMPORAL
+        fused = np.zeros(20000, bool)
+        fused[H.cmds_u32(cmds)[:cnt, 0]] = True
+        assert 0.1 * 20000 < cnt < 0.9 * 20000, cnt
+        plain = (unfused_hiz_level(objs, xfs, surfs, v) % 2 == 0) & unfused_frustum(objs, xfs, surfs, v)
+        diff = int(np.count_nonzero(plain != fused))
+        assert 100 < diff < 2000, diff
+
+
 def test_boundary_scene_separates_the_fused_points():
     """The objects of tests.helpers.boundary_scene sit on the frustum's side planes, so their decisions depend on
     where the arithmetic rounds: about half are inside, and the oracle (fused multiply-adds at its fixed points) and
