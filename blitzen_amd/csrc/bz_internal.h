@@ -93,12 +93,22 @@ struct FrameCacheDev {
 // survivors there.
 hipError_t launch_cull(const CullJob& job, const CompactScratch& s, uint32_t block, uint32_t items, uint32_t n_cu,
                        uint32_t per_cu, hipStream_t st, const FrameCacheDev* cache = nullptr);
+// Chunks of consecutive tiles: chunk c of a job holds tiles [t0, t1), the first n_long chunks tpc + 1 tiles and the
+// others tpc (so that as many chunks as workgroups fit at once take the whole job evenly).
+__host__ __device__ inline void chunk_range(uint32_t c, uint32_t tpc, uint32_t n_long, uint32_t n_tiles, uint32_t& t0,
+                                            uint32_t& t1)
+{
+    t0 = c * tpc + (c < n_long ? c : n_long);
+    t1 = t0 + tpc + (c < n_long ? 1u : 0u);
+    if (t1 > n_tiles) t1 = n_tiles;
+    if (t0 > t1) t0 = t1;
+}
 // LOD histogram per chunk of decided tiles, filled by the decide kernel of the LOD-instanced output: workgroup c
-// decides tiles [c * tiles_per_chunk, (c + 1) * tiles_per_chunk) in order, counts its emitting objects per LOD in LDS
+// decides the tiles of chunk c (chunk_range) in order, counts its emitting objects per LOD in LDS
 // (n_lods <= kInstLdsBins) and writes the counts to hist[c * n_lods + LOD].
 struct InstHistDev {
     uint32_t* hist;
-    uint32_t tiles_per_chunk, n_lods;
+    uint32_t tiles_per_chunk, n_long, n_lods;
 };
 // The decide kernel alone: per tile, the emitting objects' (objectId, LOD) in ascending objectId and their count
 // (PLAIN decisions), and with `ih` their LOD histogram per chunk of tiles.
@@ -114,7 +124,7 @@ struct InstancedJob {
     uint32_t* counters;           // [n_lods] x 2
     uint32_t* cmds;               // [capacity] x 8
     uint32_t* count;
-    uint32_t n_lods, per_lod, capacity, n_tiles, tile_size, tiles_per_chunk, n_chunks;
+    uint32_t n_lods, per_lod, capacity, n_tiles, tile_size, tiles_per_chunk, n_long, n_chunks;  // chunk_range
     uint32_t* err;                // [0] set when the scan gives up its bounded wait
     uint32_t spin_limit;
 };
@@ -133,7 +143,7 @@ hipError_t launch_instanced(const InstancedJob& job, const CompactScratch& s, ui
 // The decide of the LOD-instanced output with its LDS histogram: the workgroup count and tiles per workgroup
 // (= chunk) it will use for `job`, and the launch.
 void decide_inst_chunks(const CullJob& job, uint32_t block, uint32_t items, uint32_t n_cu, uint32_t per_cu,
-                        uint32_t* tiles_per_chunk, uint32_t* n_chunks);
+                        uint32_t* tiles_per_chunk, uint32_t* n_long, uint32_t* n_chunks);
 
 // LATE pass from a frame cache recorded with the same tile size (block x items) over the same range. It runs over
 // super-tiles of kSuperTile objects and needs late_cached_tiles(n) counts and late_cached_entries(n) staging slots.

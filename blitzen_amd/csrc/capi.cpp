@@ -1023,10 +1023,12 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
     if (job.n) {
         ij.n_tiles = job.n_tiles;
         if (fused) {
-            decide_inst_chunks(job, t.block, t.items, ctx->n_cu, t.grid_per_cu, &ij.tiles_per_chunk, &ij.n_chunks);
+            decide_inst_chunks(job, t.block, t.items, ctx->n_cu, t.grid_per_cu, &ij.tiles_per_chunk, &ij.n_long,
+                               &ij.n_chunks);
         } else {
             const uint32_t target = std::max<uint32_t>(1u, std::min<uint32_t>(512u, (1u << 24) / s->n_lods));
             ij.tiles_per_chunk = (job.n_tiles + target - 1) / target;
+            ij.n_long = 0;
             ij.n_chunks = (job.n_tiles + ij.tiles_per_chunk - 1) / ij.tiles_per_chunk;
         }
     }
@@ -1060,7 +1062,7 @@ bz_status bz_cull_instanced(bz_ctx* ctx, const bz_instanced_cull_args* a, void* 
     ij.err = sc->err;
     ij.spin_limit = ctx->wait_limit;
     if (job.n) {
-        const InstHistDev ih{ sc->inst_hist, ij.tiles_per_chunk, ij.n_lods };
+        const InstHistDev ih{ sc->inst_hist, ij.tiles_per_chunk, ij.n_long, ij.n_lods };
         hipError_t e = launch_decide_only(job, sc->cs, t.block, t.items, ctx->n_cu, t.grid_per_cu, st,
                                           fused ? &ih : nullptr);
         if (e != hipSuccess) return ctx->hip_fail(e, "decide launch");

@@ -76,8 +76,8 @@ __global__ __launch_bounds__(256) void inst_hist_kernel(InstancedJob J, const ui
     const uint32_t l1 = l0 + kSlice < J.n_lods ? l0 + kSlice : J.n_lods;
     for (uint32_t i = tid; i < l1 - l0; i += 256) h[i] = 0u;
     __syncthreads();
-    const uint32_t c0 = c * J.tiles_per_chunk;
-    const uint32_t c1 = c0 + J.tiles_per_chunk < J.n_tiles ? c0 + J.tiles_per_chunk : J.n_tiles;
+    uint32_t c0, c1;
+    chunk_range(c, J.tiles_per_chunk, J.n_long, J.n_tiles, c0, c1);
     for (uint32_t t0 = c0; t0 < c1; t0 += 64) {
         const uint32_t nt = c1 - t0 < 64u ? c1 - t0 : 64u;
         const TileWindow w = load_window(counts, t0, nt, lane);
@@ -285,8 +285,8 @@ __global__ __launch_bounds__(kScatterWaves * 64) void inst_scatter_kernel(Instan
             if (i < nl) running[i] = v[k];
         }
     }
-    const uint32_t c0 = c * J.tiles_per_chunk;
-    const uint32_t c1 = c0 + J.tiles_per_chunk < J.n_tiles ? c0 + J.tiles_per_chunk : J.n_tiles;
+    uint32_t c0, c1;
+    chunk_range(c, J.tiles_per_chunk, J.n_long, J.n_tiles, c0, c1);
     for (uint32_t t0 = c0; t0 < c1; t0 += 64) {
         const uint32_t nt = c1 - t0 < 64u ? c1 - t0 : 64u;
         const TileWindow w = load_window(counts, t0, nt, lane);
