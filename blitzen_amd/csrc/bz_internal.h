@@ -165,8 +165,9 @@ inline uint32_t max_job_tiles(const CullJob* jobs, size_t n)
 }
 hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st);
 // starts_dev: the jobs' wg_start values as a compact ascending array (the batched kernels stage it in LDS)
-hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
-                                uint32_t total_wgs, uint32_t* err, hipStream_t st);
+// Many pyramid builds in one launch, one workgroup per (level-0 tile, job); max_tiles = the largest job's tiles.
+hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, uint32_t n_jobs, uint32_t max_tiles, uint32_t* err,
+                                hipStream_t st);
 hipError_t launch_pyramid_level(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
                                 hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t st);

@@ -863,8 +863,9 @@ bz_status bz_pyramid_build_batch(bz_ctx* ctx, const bz_pyramid_job* jobs, uint32
     if (s) return s;
     s = ctx->upload_pjobs(sc, sp.data(), (uint32_t)sp.size(), st);
     if (s) return s;
-    hipError_t e = launch_pyramid_batch(sc->pjobs_dev, table_starts(sc->pjobs_dev, sc->pjobs_cap), (uint32_t)sp.size(),
-                                        wg, sc->err + 1, st);
+    uint32_t mt = 0;
+    for (auto& q : sp) mt = std::max<uint32_t>(mt, q.tiles_x * q.tiles_y);
+    hipError_t e = launch_pyramid_batch(sc->pjobs_dev, (uint32_t)sp.size(), mt, sc->err + 1, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "pyramid batch launch");
     return BZ_OK;
@@ -1369,8 +1370,11 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     if (e != hipSuccess) return ctx->hip_fail(e, "capture stream");
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess && !pj.empty())
-        e = launch_pyramid_batch(g->pjobs_dev, table_starts(g->pjobs_dev, (uint32_t)pj.size()), (uint32_t)pj.size(), wg,
-                                 g->err + 1, cs);
+    {
+        uint32_t mt = 0;
+        for (auto& q : pj) mt = std::max<uint32_t>(mt, q.tiles_x * q.tiles_y);
+        e = launch_pyramid_batch(g->pjobs_dev, (uint32_t)pj.size(), mt, g->err + 1, cs);
+    }
     for (auto& m : multi) {
         if (e != hipSuccess) break;
         Pyramid* p = m.first;

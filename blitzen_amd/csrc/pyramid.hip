@@ -259,29 +259,22 @@ __device__ __forceinline__ uint32_t last_le(const uint32_t* starts, uint32_t n, 
     return lo;
 }
 
-// Many pyramids in one launch: workgroup b builds tile b - wg_start of the job whose range holds b. The compact
-// table of first workgroups is staged in LDS (one coalesced read) and the job descriptor is copied to LDS.
-__global__ __launch_bounds__(kPyrBlock) void pyramid_batch_kernel(const PyrBuildJob* __restrict__ jobs,
-                                                                  const uint32_t* __restrict__ starts, uint32_t n_jobs,
-                                                                  uint32_t* err)
+// Many pyramids in one launch: workgroup (x, y) builds level-0 tile x of job y; workgroups past the job's tiles return
+// at once. The job index is the workgroup's y, so no table of first workgroups is searched (the search's load cost
+// each workgroup a round trip: the 256-view batch 336.6 us against 314.6 us); the descriptor is copied to LDS. Within
+// a job, workgroups are dispatched in tile order, so the job's last tile (which waits on the others' hand-off words)
+// starts after every tile it waits for.
+__global__ __launch_bounds__(kPyrBlock) void pyramid_batch_kernel(const PyrBuildJob* __restrict__ jobs, uint32_t* err)
 {
     __shared__ float bufA[kTileTexels];
     __shared__ float bufB[kTileTexels / 2];
-    __shared__ uint32_t s_start[kLdsJobs];
     __shared__ PyrBuildJob sJ;
-    uint32_t j;
-    if (n_jobs <= kLdsJobs) {
-        for (uint32_t k = threadIdx.x; k < n_jobs; k += kPyrBlock) s_start[k] = starts[k];
-        __syncthreads();
-        j = last_le(s_start, n_jobs, blockIdx.x);
-    } else {
-        j = last_le(starts, n_jobs, blockIdx.x);
-    }
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[j]);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&jobs[blockIdx.y]);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&sJ);
     for (uint32_t w = threadIdx.x; w < sizeof(PyrBuildJob) / 4; w += kPyrBlock) dst[w] = src[w];
     __syncthreads();
-    pyramid_tile<kPyrBlock>(sJ, blockIdx.x - sJ.wg_start, bufA, bufB, err);
+    if (blockIdx.x >= sJ.tiles_x * sJ.tiles_y) return;
+    pyramid_tile<kPyrBlock>(sJ, blockIdx.x, bufA, bufB, err);
 }
 
 // General per-level footprint reduction (any sizes).
@@ -398,12 +391,17 @@ hipError_t launch_pyramid(const PyrBuildJob& job, uint32_t* err, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs,
-                                uint32_t total_wgs, uint32_t* err, hipStream_t st)
+hipError_t launch_pyramid_batch(const PyrBuildJob* jobs_dev, uint32_t n_jobs, uint32_t max_tiles, uint32_t* err,
+                                hipStream_t st)
 {
-    if (n_jobs == 0 || total_wgs == 0) return hipSuccess;
-    hipLaunchKernelGGL(pyramid_batch_kernel, dim3(total_wgs), dim3(kPyrBlock), 0, st, jobs_dev, starts_dev, n_jobs,
-                       err);
+    if (n_jobs == 0 || max_tiles == 0) return hipSuccess;
+    constexpr uint32_t kMaxGridY = 65535;  // jobs per launch (the grid's y extent); more go in further launches
+    for (uint32_t j0 = 0; j0 < n_jobs; j0 += kMaxGridY) {
+        const uint32_t n = n_jobs - j0 < kMaxGridY ? n_jobs - j0 : kMaxGridY;
+        hipLaunchKernelGGL(pyramid_batch_kernel, dim3(max_tiles, n), dim3(kPyrBlock), 0, st, jobs_dev + j0, err);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 

@@ -233,8 +233,11 @@ bz_status HipDevice::launch(uint32_t slot, const std::vector<ReqP>& batch)
     if (e == hipSuccess)
         e = launch_synth_depth_batch(s.sjobs_dev, table_starts(s.sjobs_dev, s.sjobs_cap), B, swg, s.st);
     if (e == hipSuccess && !pj.empty())
-        e = launch_pyramid_batch(s.pjobs_dev, table_starts(s.pjobs_dev, s.pjobs_cap), (uint32_t)pj.size(), pwg,
-                                 s.err + 1, s.st);
+    {
+        uint32_t mt = 0;
+        for (auto& q : pj) mt = std::max<uint32_t>(mt, q.tiles_x * q.tiles_y);
+        e = launch_pyramid_batch(s.pjobs_dev, (uint32_t)pj.size(), mt, s.err + 1, s.st);
+    }
     if (e == hipSuccess && !cj.empty())
         e = launch_cull_batch(s.jobs_dev, table_starts(s.jobs_dev, s.jobs_cap), (uint32_t)cj.size(), tiles,
                               max_job_tiles(cj.data(), cj.size()), s.cs, t.block, t.items, s.st);
