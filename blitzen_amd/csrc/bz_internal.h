@@ -152,11 +152,18 @@ inline uint32_t late_cached_tiles(uint32_t n) { return (uint32_t)(((uint64_t)n +
 inline uint64_t late_cached_entries(uint32_t n) { return (uint64_t)late_cached_tiles(n) * kSuperTile; }
 hipError_t launch_late_cached(const CullJob& job, const CompactScratch& s, const FrameCacheDev& cache, uint32_t block,
                               uint32_t items, hipStream_t st);
+// The first tiles of up to kArgJobs jobs, passed to the batched decide as a kernel argument (2 KiB of the 4 KiB
+// kernarg segment).
+constexpr uint32_t kArgJobs = 512;
+struct JobStartsArg {
+    uint32_t v[kArgJobs];
+};
+// jobs_host: the same n_jobs jobs in host memory (their tile_start values fill the kernel-argument table)
 // starts_dev: the jobs' tile_start values as a compact ascending array (the batched kernels stage it in LDS)
 // max_job_tiles: the largest job's tile count (above kScanTiles the emit reads a scanned prefix of the batch)
-hipError_t launch_cull_batch(const CullJob* jobs_dev, const uint32_t* starts_dev, uint32_t n_jobs, uint32_t total_tiles,
-                             uint32_t max_job_tiles, const CompactScratch& s, uint32_t block, uint32_t items,
-                             hipStream_t st);
+hipError_t launch_cull_batch(const CullJob* jobs_host, const CullJob* jobs_dev, const uint32_t* starts_dev,
+                             uint32_t n_jobs, uint32_t total_tiles, uint32_t max_job_tiles, const CompactScratch& s,
+                             uint32_t block, uint32_t items, hipStream_t st);
 inline uint32_t max_job_tiles(const CullJob* jobs, size_t n)
 {
     uint32_t m = 0;

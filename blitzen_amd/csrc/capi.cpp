@@ -1230,8 +1230,8 @@ bz_status bz_cull_batch(bz_ctx* ctx, const bz_view_job* jobs, uint32_t n_jobs, v
     if (s) return s;
     s = ctx->upload_jobs(sc, cj.data(), (uint32_t)cj.size(), st);
     if (s) return s;
-    e = launch_cull_batch(sc->jobs_dev, table_starts(sc->jobs_dev, sc->jobs_cap), (uint32_t)cj.size(), tiles,
-                          max_job_tiles(cj.data(), cj.size()), sc->cs, t.block, t.items, st);
+    e = launch_cull_batch(cj.data(), sc->jobs_dev, table_starts(sc->jobs_dev, sc->jobs_cap), (uint32_t)cj.size(),
+                          tiles, max_job_tiles(cj.data(), cj.size()), sc->cs, t.block, t.items, st);
     if (e == hipSuccess) e = hipEventRecord(sc->upload_done, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "batched cull launch");
     return BZ_OK;
@@ -1386,8 +1386,9 @@ bz_status bz_batch_graph_create(bz_ctx* ctx, const bz_pyramid_job* pyr_jobs, uin
     }
     if (e == hipSuccess) e = zero_counts(empty, cs);
     if (e == hipSuccess && !cj.empty())
-        e = launch_cull_batch(g->jobs_dev, table_starts(g->jobs_dev, (uint32_t)cj.size()), (uint32_t)cj.size(), tiles,
-                              max_job_tiles(cj.data(), cj.size()), g->cs, t.block, t.items, cs);
+        e = launch_cull_batch(cj.data(), g->jobs_dev, table_starts(g->jobs_dev, (uint32_t)cj.size()),
+                              (uint32_t)cj.size(), tiles, max_job_tiles(cj.data(), cj.size()), g->cs, t.block,
+                              t.items, cs);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     (void)hipStreamDestroy(cs);

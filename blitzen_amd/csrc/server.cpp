@@ -239,8 +239,8 @@ bz_status HipDevice::launch(uint32_t slot, const std::vector<ReqP>& batch)
         e = launch_pyramid_batch(s.pjobs_dev, (uint32_t)pj.size(), mt, s.err + 1, s.st);
     }
     if (e == hipSuccess && !cj.empty())
-        e = launch_cull_batch(s.jobs_dev, table_starts(s.jobs_dev, s.jobs_cap), (uint32_t)cj.size(), tiles,
-                              max_job_tiles(cj.data(), cj.size()), s.cs, t.block, t.items, s.st);
+        e = launch_cull_batch(cj.data(), s.jobs_dev, table_starts(s.jobs_dev, s.jobs_cap), (uint32_t)cj.size(),
+                              tiles, max_job_tiles(cj.data(), cj.size()), s.cs, t.block, t.items, s.st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s.counts_host, s.counts_dev, (size_t)B * 4, hipMemcpyDeviceToHost, s.st);
     if (e == hipSuccess)
