@@ -131,8 +131,9 @@ struct InstancedJob {
 // The LOD-instanced output's device state: `hist`, the chunk x LOD histogram (n_chunks x n_lods words, every word
 // written by the decide kernel or the histogram kernel), and `seg`, inst_seg_words() 64-bit words in which the scan
 // publishes each segment of kInstSegChunks chunks' totals with a presence flag; zero between calls (the scatter
-// launch clears them).
-constexpr uint32_t kInstSegChunks = 32;
+// launch clears them). 64 chunks per segment: at 767 chunks the scan took 5.7 us against 5.9 us with 32 and 7.1 us
+// with 16, and the scatter clears half as many words.
+constexpr uint32_t kInstSegChunks = 64;
 inline uint32_t inst_segments(uint32_t n_chunks) { return n_chunks ? (n_chunks + kInstSegChunks - 1) / kInstSegChunks : 1u; }
 inline uint64_t inst_seg_words(uint32_t n_chunks, uint32_t n_lods) { return (uint64_t)inst_segments(n_chunks) * n_lods; }
 // LODs the decide kernel can histogram in LDS (more: a separate histogram kernel reads the decided entries).

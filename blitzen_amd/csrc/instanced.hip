@@ -9,7 +9,7 @@
 //            LDS when the LODs fit kInstLdsBins (a chunk is then one workgroup's contiguous run of tiles, and the
 //            decided entries are read once, by the scatter); otherwise one workgroup per chunk x LOD slice builds an
 //            LDS histogram from the decided entries (loaded as 64-entry groups, kBatch groups per round trip);
-//   scan:    one workgroup per 64 LODs x segment of 32 chunks: exclusive prefix of the histogram over the chunks (the
+//   scan:    one workgroup per 64 LODs x segment of 64 chunks: exclusive prefix of the histogram over the chunks (the
 //            chunk's first slot in the LOD's list; the segments' totals pass between workgroups by look-back) and the
 //            counter {lod * per_lod, total};
 //   cmds:    one extra workgroup of the scatter launch: the LODs with instances in ascending order -> 32-byte commands
