@@ -147,6 +147,26 @@ def test_batch_matches_oracle(ctx, scenes):
         b.free()
 
 
+@pytest.mark.parametrize("n_views", [600, 1100])
+def test_batch_many_views(ctx, scenes, n_views):
+    """Batches past the job lookups' kernel-argument table (512 views): the first tiles staged in LDS (up to 1024
+    views) and searched in global memory (beyond), each job a short range of its own."""
+    sh, sc = scenes[0]
+    n = len(sc[0])
+    rng = np.random.default_rng(n_views)
+    jobs = []
+    for k in range(n_views):
+        count = int(rng.integers(1, 2600))
+        begin = int(rng.integers(0, n - count))
+        jobs.append(Job(ctx, sh, sc, H.view(yaw=float(rng.uniform(-3, 3)), pitch=float(rng.uniform(-0.5, 0.5))),
+                        A.PASS_PLAIN, begin=begin, count=count))
+    ctx.cull_batch([j.vj for j in jobs])
+    ctx.check_kernels()
+    for j in jobs:
+        j.check(ctx, None)
+        j.free()
+
+
 @pytest.mark.parametrize("tuning", [(256, 2), (256, 4), (512, 2), (512, 4)])
 def test_batch_tunings(ctx, scenes, tuning):
     old = ctx.get_tuning(1)
