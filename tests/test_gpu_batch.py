@@ -147,10 +147,12 @@ def test_batch_matches_oracle(ctx, scenes):
         b.free()
 
 
+@pytest.mark.parametrize("mode", ["eager", "graph"])
 @pytest.mark.parametrize("n_views", [600, 1100])
-def test_batch_many_views(ctx, scenes, n_views):
+def test_batch_many_views(ctx, scenes, n_views, mode):
     """Batches past the job lookups' kernel-argument table (512 views): the first tiles staged in LDS (up to 1024
-    views) and searched in global memory (beyond), each job a short range of its own."""
+    views) and searched in global memory (beyond), each job a short range of its own; run eagerly, or captured in a
+    graph and replayed twice (the counts poisoned before the second replay)."""
     sh, sc = scenes[0]
     n = len(sc[0])
     rng = np.random.default_rng(n_views)
@@ -160,11 +162,29 @@ def test_batch_many_views(ctx, scenes, n_views):
         begin = int(rng.integers(0, n - count))
         jobs.append(Job(ctx, sh, sc, H.view(yaw=float(rng.uniform(-3, 3)), pitch=float(rng.uniform(-0.5, 0.5))),
                         A.PASS_PLAIN, begin=begin, count=count))
-    ctx.cull_batch([j.vj for j in jobs])
-    ctx.check_kernels()
-    for j in jobs:
-        j.check(ctx, None)
-        j.free()
+    try:
+        if mode == "eager":
+            ctx.cull_batch([j.vj for j in jobs])
+            ctx.check_kernels()
+            for j in jobs:
+                j.check(ctx, None)
+        else:
+            g = ctx.batch_graph_create([], [j.vj for j in jobs])
+            try:
+                for rep in range(2):
+                    if rep:
+                        for j in jobs:
+                            j.cnt.fill_u32(0xDEADBEEF, 1)
+                        ctx.sync()
+                    ctx.graph_launch(g)
+                    ctx.check_kernels()
+                    for j in jobs:
+                        j.check(ctx, None)
+            finally:
+                ctx.graph_destroy(g)
+    finally:
+        for j in jobs:
+            j.free()
 
 
 @pytest.mark.parametrize("tuning", [(256, 2), (256, 4), (512, 2), (512, 4)])
